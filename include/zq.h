@@ -1,0 +1,108 @@
+/*
+ * zq.h -- C ABI of libzquatev_amd, the MI355X (gfx950) quaternion-Hermitian
+ * eigensolver.  Plain pointers and sizes only.
+ *
+ * The problem (zquatev.h:41-52 of the reference): diagonalise the 2n x 2n
+ * Hermitian matrix
+ *
+ *        H = [ A  -conj(B) ]      A Hermitian, B complex antisymmetric (B^T = -B)
+ *            [ B   conj(A) ]
+ *
+ * given its left half, and return the n distinct eigenvalues (each is doubly
+ * degenerate in H) and the Kramers-paired eigenvectors [[U, -conj V], [V, conj U]].
+ *
+ * Matrix arguments are complex128 arrays passed as double* (interleaved real,
+ * imaginary), column-major with a leading dimension counted in complex
+ * elements, exactly as std::complex<double>* in the reference.
+ *
+ * Return codes: 0 success; -i argument i is invalid (LAPACK style); > 0 the
+ * tridiagonal eigensolver did not converge; ZQ_ERR_* below for runtime errors.
+ */
+#ifndef ZQ_H
+#define ZQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZQ_OK 0
+#define ZQ_ERR_HIP (-1001)    /* a HIP runtime call failed */
+#define ZQ_ERR_NOMEM (-1002)  /* device memory exhausted */
+#define ZQ_ERR_ARG (-1003)    /* unsupported internal configuration */
+
+typedef struct zq_context zq_context;
+
+/* Library version as 10000*major + 100*minor + patch. */
+int zq_version(void);
+
+/* Human-readable text for a return code. */
+const char* zq_error_string(int code);
+
+/* Create a context bound to HIP device `device`: one stream (created here) and
+ * a device workspace cached across calls.  No counterpart in the reference,
+ * whose entry point allocates per call (zquatev.cc:63-66). */
+int zq_create(zq_context** ctx, int device);
+int zq_destroy(zq_context* ctx);
+
+/* Use the caller's HIP stream (a hipStream_t passed as void*; NULL restores
+ * the context's own stream).  The device entry points are asynchronous on it. */
+int zq_set_stream(zq_context* ctx, void* stream);
+
+/* Panel width (quaternion columns per panel, 1..128).  0 selects the tuned
+ * table.  The reference hard-codes nb = 20 (zquatev.cc:64). */
+int zq_set_panel_width(zq_context* ctx, int nb);
+
+/* Panel width the tuned table picks for half-dimension n. */
+int zq_tuned_panel_width(int n);
+
+/* Drop-in semantics of ts::zquatev (zquatev.h:54, zquatev.cc:42-100) on host
+ * memory.  D: nld2 x n2 complex128 (the full buffer is used; only the left
+ * half is read on input).  eig: n2/2 doubles, ascending on output.  On return
+ * D holds [[U, -conj V], [V, conj U]]; columns i and i + n2/2 are the Kramers
+ * pair of eig[i].  Synchronous.  Uses a process-wide context on device 0. */
+int zq_zquatev(int n2, double* D, int nld2, double* eig);
+
+/* Same on the given context (host memory, synchronous). */
+int zq_zquatev_ctx(zq_context* ctx, int n2, double* D, int nld2, double* eig);
+
+/* Device-memory variant: dD (device, nld2 x n2 complex128), deig (device,
+ * n2/2 doubles).  jobz = 'V' computes eigenvectors into dD as above; jobz =
+ * 'N' computes eigenvalues only and leaves dD unchanged.  Asynchronous on the
+ * context's stream. */
+int zq_zquatev_device(zq_context* ctx, int n2, double* dD, int nld2, double* deig, int jobz);
+
+/* Component entry points (device memory, asynchronous).
+ *
+ * zq_tridiagonalize: the symplectic reduction of the reference's panel loop
+ * (zquatev.cc:68-72 over impl::panel_update, blocked.cc:45, and
+ * impl::unblocked_update, unblocked.cc:44).  Reads the left half of dD and
+ * writes the real tridiagonal it reduces to: dd (n doubles, diagonal) and
+ * de (n-1 doubles, subdiagonal magnitudes) -- the (phase-scaled) matrix the
+ * reference passes to zhbev at zquatev.cc:79-84.
+ */
+int zq_tridiagonalize(zq_context* ctx, int n2, const double* dD, int nld2, double* dd, double* de);
+
+/* zq_tridiag_eig: eigenpairs of the real symmetric tridiagonal (dd, de) of
+ * order n, the tridiagonal step of zquatev.cc:74-84 (zhbev_ with kd = 1, after
+ * its phase scaling).  dw: n eigenvalues ascending; dZ (n x n, ld n, may be
+ * NULL for eigenvalues only): orthonormal eigenvectors. */
+int zq_tridiag_eig(zq_context* ctx, int n, const double* dd, const double* de, double* dw, double* dZ);
+
+/* Device-side timings of the last solve on ctx, in milliseconds:
+ * ms[0] reduction, ms[1] tridiagonal eigensolve, ms[2] back-transform,
+ * ms[3] layout in/out, ms[4] total.  Returns the number of values written. */
+int zq_last_timings(zq_context* ctx, double* ms, int count);
+
+/* Self-test of the MFMA register layout the kernels rely on: out (64 x 4
+ * doubles, device) = raw accumulator of one v_mfma_f64_16x16x4_f64 with
+ * A and B (64 doubles each, device) as the per-lane operands. */
+int zq_selftest_mfma(zq_context* ctx, const double* dA, const double* dB, double* dOut);
+
+/* Stream synchronisation helper for callers without their own HIP runtime binding. */
+int zq_synchronize(zq_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZQ_H */

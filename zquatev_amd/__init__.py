@@ -1,0 +1,221 @@
+"""MI355X (gfx950) quaternion-Hermitian eigensolver -- Python binding of the C ABI.
+
+``zquatev(D)`` mirrors the reference entry point ``ts::zquatev(n2, D, ld, eig)``
+(zquatev.h:54): D is a column-major (Fortran-order) complex128 array of shape
+(2n, 2n) whose left half holds [A; B]; on return D holds [[U, -conj V],
+[V, conj U]] and the function returns (info, eig) with eig ascending.
+
+All arithmetic runs in libzquatev_amd.so (HIP kernels for gfx950).  There is
+no CPU fallback: every entry point raises ImportError if the library is missing.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzquatev_amd.so")
+
+_c_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes) of every C-ABI entry point in include/zq.h
+_SIGNATURES = {
+    "zq_version": (_c_int, []),
+    "zq_error_string": (ctypes.c_char_p, [_c_int]),
+    "zq_create": (_c_int, [ctypes.POINTER(_vp), _c_int]),
+    "zq_destroy": (_c_int, [_vp]),
+    "zq_set_stream": (_c_int, [_vp, _vp]),
+    "zq_set_panel_width": (_c_int, [_vp, _c_int]),
+    "zq_tuned_panel_width": (_c_int, [_c_int]),
+    "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
+    "zq_zquatev_ctx": (_c_int, [_vp, _c_int, _vp, _c_int, _vp]),
+    "zq_zquatev_device": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int]),
+    "zq_tridiagonalize": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
+    "zq_tridiag_eig": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
+    "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
+    "zq_selftest_mfma": (_c_int, [_vp, _vp, _vp, _vp]),
+    "zq_synchronize": (_c_int, [_vp]),
+}
+
+_LIB = None
+
+
+class _Fns:
+    pass
+
+
+_F = _Fns()
+
+
+def lib():
+    """Load libzquatev_amd.so (once).  Raises ImportError if it was not built:
+    there is no CPU fallback."""
+    global _LIB
+    if _LIB is None:
+        # PyTorch ships its own HIP runtime under the same soname; when torch is
+        # present it must be loaded first so that the process has one runtime
+        # and torch's device pointers are valid here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(make -C zquatev_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+            setattr(_F, name, f)
+        _LIB = L
+    return _LIB
+
+
+def _fn(name):
+    lib()
+    return getattr(_F, name)
+
+
+ERR_HIP = -1001
+ERR_NOMEM = -1002
+ERR_ARG = -1003
+
+
+class ZQError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        msg = _fn("zq_error_string")(code).decode()
+        super().__init__(f"{where}: {msg} (code {code})" if where else f"{msg} (code {code})")
+
+
+def version():
+    return _fn("zq_version")()
+
+
+def error_string(code):
+    return _fn("zq_error_string")(code).decode()
+
+
+def tuned_panel_width(n):
+    return _fn("zq_tuned_panel_width")(n)
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise ZQError(rc, where)
+
+
+def _fortran_c128(D):
+    if not isinstance(D, np.ndarray) or D.dtype != np.complex128 or D.ndim != 2:
+        raise TypeError("D must be a 2-D complex128 numpy array")
+    if not D.flags.f_contiguous:
+        raise ValueError("D must be column-major (Fortran order)")
+    return D
+
+
+def zquatev(D, n2=None):
+    """In-place drop-in for ts::zquatev on host memory (zquatev.cc:42).
+
+    D: (ld2, >= n2) complex128, Fortran order; the left half of the leading
+    n2 x n2 block holds [A; B].  Returns (info, eig[n2/2])."""
+    D = _fortran_c128(D)
+    if n2 is None:
+        n2 = D.shape[1]
+    ld2 = D.shape[0]
+    eig = np.zeros(max(n2 // 2, 1), dtype=np.float64)
+    info = _fn("zq_zquatev")(n2, D.ctypes.data, ld2, eig.ctypes.data)
+    return info, eig[: n2 // 2]
+
+
+class Context:
+    """A device context: one HIP stream and a cached device workspace."""
+
+    def __init__(self, device=0, panel_width=0):
+        h = _vp()
+        _check(_fn("zq_create")(ctypes.byref(h), device), "zq_create")
+        self._h = h
+        self.device = device
+        if panel_width:
+            self.set_panel_width(panel_width)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _fn("zq_destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_stream(self, stream_ptr):
+        _check(_fn("zq_set_stream")(self._h, _vp(stream_ptr or 0)), "zq_set_stream")
+
+    def set_panel_width(self, nb):
+        _check(_fn("zq_set_panel_width")(self._h, int(nb)), "zq_set_panel_width")
+
+    def synchronize(self):
+        _check(_fn("zq_synchronize")(self._h), "zq_synchronize")
+
+    def zquatev(self, D, n2=None):
+        """Host-memory solve on this context; same contract as :func:`zquatev`."""
+        D = _fortran_c128(D)
+        if n2 is None:
+            n2 = D.shape[1]
+        eig = np.zeros(max(n2 // 2, 1), dtype=np.float64)
+        info = _fn("zq_zquatev_ctx")(self._h, n2, D.ctypes.data, D.shape[0], eig.ctypes.data)
+        return info, eig[: n2 // 2]
+
+    # device-pointer entry points: pointers are integers (e.g. torch .data_ptr())
+    def zquatev_device(self, n2, d_ptr, ld2, eig_ptr, jobz="V"):
+        return _fn("zq_zquatev_device")(self._h, int(n2), _vp(d_ptr), int(ld2), _vp(eig_ptr), ord(jobz))
+
+    def tridiagonalize_device(self, n2, d_ptr, ld2, dd_ptr, de_ptr):
+        return _fn("zq_tridiagonalize")(self._h, int(n2), _vp(d_ptr), int(ld2), _vp(dd_ptr), _vp(de_ptr))
+
+    def tridiag_eig_device(self, n, dd_ptr, de_ptr, dw_ptr, dz_ptr=0):
+        return _fn("zq_tridiag_eig")(self._h, int(n), _vp(dd_ptr), _vp(de_ptr), _vp(dw_ptr), _vp(dz_ptr or 0))
+
+    def selftest_mfma_device(self, a_ptr, b_ptr, out_ptr):
+        return _fn("zq_selftest_mfma")(self._h, _vp(a_ptr), _vp(b_ptr), _vp(out_ptr))
+
+    def last_timings(self):
+        """Device phase times of the last solve (ms): reduce, tridiagonal, back-transform, layout, total."""
+        buf = (ctypes.c_double * 5)()
+        k = _fn("zq_last_timings")(self._h, buf, 5)
+        keys = ("reduce", "tridiag", "backtransform", "layout", "total")
+        return {keys[i]: buf[i] for i in range(k)}
+
+
+# torch helpers (torch is plumbing for device memory and streams only)
+def _torch():
+    import torch
+    return torch
+
+
+def solve_torch(ctx, D, jobz="V"):
+    """Solve on a device tensor D (2n x 2n complex128, column-major, i.e. the
+    transpose of a contiguous row-major tensor is accepted as D.T).  Returns
+    (info, eig tensor); D is overwritten with the eigenvectors for jobz='V'."""
+    torch = _torch()
+    n2 = D.shape[0]
+    assert D.dtype == torch.complex128 and D.is_cuda
+    assert D.stride(0) == 1, "D must be column-major (use a transposed contiguous tensor)"
+    ld2 = D.stride(1)
+    eig = torch.zeros(max(n2 // 2, 1), dtype=torch.float64, device=D.device)
+    rc = ctx.zquatev_device(n2, D.data_ptr(), ld2, eig.data_ptr(), jobz)
+    return rc, eig[: n2 // 2]

@@ -1,0 +1,611 @@
+// Real symmetric tridiagonal eigensolver on the device: Cuppen's divide and
+// conquer with Gu-Eisenstat eigenvectors, and Sturm bisection for the
+// eigenvalues-only path.
+//
+// Replaces the tridiagonal step of the reference, zhbev_("V","L",n,1,...)
+// (zquatev.cc:74-84; MKL: zhbtrd phase scaling + zsteqr QL/QR).  The phase
+// scaling is done on the quaternion subdiagonal (backtr.hip); this file sees
+// the real tridiagonal (d, |e|) and returns the eigenvalues ascending, as
+// zhbev does, with an orthonormal real eigenvector matrix.
+//
+// Tree: halve until a block has at most LEAF rows; each split point s couples
+// rows s-1 and s with beta = e[s-1] >= 0, and T = diag(T1', T2') + beta v v^T,
+// v = e_{s-1} + e_s, where T1', T2' have beta subtracted from the touching
+// diagonal entries.  Leaves run implicit QL in one wavefront each; merges run
+// level by level (all merges of a level in one launch):
+//   prep   : stable merge of the two sorted spectra, z = [last row of Q1;
+//            first row of Q2]/sqrt(2), rho = 2 beta, deflation scan (small
+//            |z| or close d handled by a Givens rotation), one workgroup.
+//   rotgat : apply the deflation rotations to Q's columns (row-parallel) and
+//            gather the kept and deflated columns into Qtmp.
+//   secular: one root of 1 + rho sum z_j^2/(d_j - lambda) per thread, by
+//            bisection on the offset from the nearer pole.
+//   zhat   : Gu-Eisenstat z from the computed roots, for orthogonality.
+//   buildU : U[:, i] = zhat / (d - lambda_i), normalised.
+//   order  : merge the roots with the deflated values (both sorted).
+//   dgemm  : Z[:, pos] = Qtmp[:, :K] U on FP64 MFMA; deflated columns copied.
+#include "zq_internal.h"
+#include <vector>
+#include <algorithm>
+#include <cfloat>
+
+namespace zq {
+
+constexpr int LEAF = 64;
+constexpr double DEPS = DBL_EPSILON * 0.5;   // unit roundoff
+
+struct MergeDesc {
+  int lo, mid, hi;
+};
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// --------------------------------------------------------------------------
+// scale: s = max(max|d|, max|e|); d, e scaled by 1/s into lam, es
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_dc_scale(int n, const double* __restrict__ d, const double* __restrict__ e,
+                                                   double* __restrict__ lam, double* __restrict__ es,
+                                                   double* __restrict__ scal) {
+  __shared__ double sh[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(d[i]));
+  for (int i = threadIdx.x; i < n - 1; i += 1024) m = fmax(m, fabs(e[i]));
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) m = fmax(m, __shfl_xor(m, k, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < 16; ++k) s = fmax(s, sh[k]);
+  if (!(s > 0.0)) s = 1.0;
+  const double inv = 1.0 / s;
+  for (int i = threadIdx.x; i < n; i += 1024) lam[i] = d[i] * inv;
+  for (int i = threadIdx.x; i < n - 1; i += 1024) es[i] = fabs(e[i]) * inv;
+  if (threadIdx.x == 0) scal[0] = s;
+}
+
+// subtract the coupling beta = es[s-1] from the diagonal entries beside each split s
+__global__ void k_dc_tear(int nsplit, const int* __restrict__ splits, const double* __restrict__ es,
+                          double* __restrict__ lam) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsplit) return;
+  const int s = splits[t];
+  const double b = es[s - 1];
+  lam[s - 1] -= b;
+  lam[s] -= b;
+}
+
+// --------------------------------------------------------------------------
+// leaf: implicit QL with Wilkinson-type shift on a <= 64 tridiagonal.  The
+// scalar recurrence runs identically in every lane; lane k owns row k of the
+// leaf's eigenvector block and applies each rotation to it.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_dc_leaf(int n, const int* __restrict__ leaves, double* __restrict__ lam,
+                                                const double* __restrict__ es, double* __restrict__ Z, int ldz,
+                                                int* __restrict__ info) {
+  __shared__ double dd[LEAF], ee[LEAF];
+  __shared__ double zz[LEAF][LEAF + 1];
+  const int lo = leaves[2 * blockIdx.x], hi = leaves[2 * blockIdx.x + 1];
+  const int m = hi - lo;
+  const int k = threadIdx.x;
+  if (k < m) {
+    dd[k] = lam[lo + k];
+    ee[k] = (k < m - 1) ? es[lo + k] : 0.0;
+    for (int i = 0; i < m; ++i) zz[k][i] = (i == k) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int fail = 0;
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    int mm;
+    while (true) {
+      for (mm = l; mm < m - 1; ++mm) {
+        const double s = fabs(dd[mm]) + fabs(dd[mm + 1]);
+        if (fabs(ee[mm]) <= DEPS * s) break;
+      }
+      if (mm == l) break;
+      if (++iter > 60) { fail = 1; break; }
+      double g = (dd[l + 1] - dd[l]) / (2.0 * ee[l]);
+      double r = hypot(g, 1.0);
+      g = dd[mm] - dd[l] + ee[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i;
+      bool early = false;
+      for (i = mm - 1; i >= l; --i) {
+        const double f = s * ee[i];
+        const double b = c * ee[i];
+        r = hypot(f, g);
+        __syncthreads();
+        if (k == 0) ee[i + 1] = r;
+        __syncthreads();
+        if (r == 0.0) {
+          __syncthreads();
+          if (k == 0) { dd[i + 1] -= p; ee[mm] = 0.0; }
+          __syncthreads();
+          early = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = dd[i + 1] - p;
+        r = (dd[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        __syncthreads();
+        if (k == 0) dd[i + 1] = g + p;
+        __syncthreads();
+        g = c * r - b;
+        if (k < m) {
+          const double zf = zz[k][i + 1];
+          const double zi = zz[k][i];
+          zz[k][i + 1] = s * zi + c * zf;
+          zz[k][i] = c * zi - s * zf;
+        }
+      }
+      if (early) continue;
+      __syncthreads();
+      if (k == 0) { dd[l] -= p; ee[l] = g; ee[mm] = 0.0; }
+      __syncthreads();
+    }
+    if (fail) break;
+  }
+  // selection sort ascending, columns follow
+  for (int i = 0; i < m - 1; ++i) {
+    int kmin = i;
+    double vmin = dd[i];
+    for (int j = i + 1; j < m; ++j)
+      if (dd[j] < vmin) { vmin = dd[j]; kmin = j; }
+    if (kmin != i) {
+      __syncthreads();
+      if (k == 0) { dd[kmin] = dd[i]; dd[i] = vmin; }
+      if (k < m) {
+        const double t = zz[k][i];
+        zz[k][i] = zz[k][kmin];
+        zz[k][kmin] = t;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (k < m) {
+    lam[lo + k] = dd[k];
+    for (int i = 0; i < m; ++i) Z[(lo + k) + (size_t)(lo + i) * ldz] = zz[k][i];
+  }
+  if (fail && k == 0) atomicMax(info, lo + 1);
+}
+
+// --------------------------------------------------------------------------
+// merge kernels
+// --------------------------------------------------------------------------
+struct MergeBufs {
+  double* dk;    // sorted d (in place modified by deflation)
+  double* zk;    // sorted z
+  int* src;      // sorted position -> local column
+  int* keep;     // kept sorted positions (K)
+  int* dfl;      // deflated sorted positions (ns - K), sorted by value
+  double* rot;   // rotations: (pj, j, c, s) as doubles
+  int* nrot;     // per merge
+  int* kcnt;     // per merge
+  double* dkk;   // kept d
+  double* zkk;   // kept z
+  double* dval;  // deflated values
+  double* rho;   // per merge
+  int* org;      // per root: pole index (into kept list)
+  double* tau;   // per root: offset from the pole
+  double* zhat;
+  int* posK;     // final column for each root
+  int* posD;     // final column for each deflated value
+  double* Qtmp;
+  double* U;
+};
+
+__device__ __forceinline__ int count_less(const double* a, int len, double v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int count_leq(const double* a, int len, double v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict__ descs, int n,
+                                                    const double* __restrict__ lam, const double* __restrict__ es,
+                                                    const double* __restrict__ Z, int ldz, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.x];
+  const int lo = md.lo, mid = md.mid, hi = md.hi;
+  const int ns = hi - lo, n1 = mid - lo, n2 = hi - mid;
+  const double beta = es[mid - 1];
+  const double rho = 2.0 * beta;
+  double* dk = B.dk + lo;
+  double* zk = B.zk + lo;
+  int* src = B.src + lo;
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    double v, zr;
+    int r;
+    if (t < n1) {
+      v = lam[lo + t];
+      r = t + count_less(lam + mid, n2, v);
+      zr = Z[(mid - 1) + (size_t)(lo + t) * ldz];
+    } else {
+      v = lam[mid + t - n1];
+      r = (t - n1) + count_leq(lam + lo, n1, v);
+      zr = Z[mid + (size_t)(lo + t) * ldz];
+    }
+    dk[r] = v;
+    zk[r] = zr * 0.70710678118654752440;
+    src[r] = t;
+  }
+  __syncthreads();
+  __shared__ double shd[4], shz[4];
+  double dm = 0.0, zm = 0.0;
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    dm = fmax(dm, fabs(dk[t]));
+    zm = fmax(zm, fabs(zk[t]));
+  }
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) {
+    dm = fmax(dm, __shfl_xor(dm, k, 64));
+    zm = fmax(zm, __shfl_xor(zm, k, 64));
+  }
+  if ((threadIdx.x & 63) == 0) { shd[threadIdx.x >> 6] = dm; shz[threadIdx.x >> 6] = zm; }
+  __syncthreads();
+  dm = fmax(fmax(shd[0], shd[1]), fmax(shd[2], shd[3]));
+  zm = fmax(fmax(shz[0], shz[1]), fmax(shz[2], shz[3]));
+  const double tol = 8.0 * DEPS * fmax(dm, rho * zm);
+  int* keep = B.keep + lo;
+  int* dfl = B.dfl + lo;
+  double* rot = B.rot + 4 * (size_t)lo;
+  __shared__ int sK;
+  if (threadIdx.x == 0) {
+    int K = 0, nd = 0, nr = 0, pj = -1;
+    for (int j = 0; j < ns; ++j) {
+      if (rho * fabs(zk[j]) <= tol) {
+        // deflated with a negligible z: insert by value (list stays sorted)
+        int q = nd++;
+        while (q > 0 && dk[dfl[q - 1]] > dk[j]) { dfl[q] = dfl[q - 1]; --q; }
+        dfl[q] = j;
+        continue;
+      }
+      if (pj < 0) { pj = j; continue; }
+      double s = zk[pj], c = zk[j];
+      const double tt = hypot(c, s);
+      const double t = dk[j] - dk[pj];
+      c /= tt;
+      s = -s / tt;
+      if (fabs(t * c * s) <= tol) {
+        zk[j] = tt;
+        zk[pj] = 0.0;
+        rot[4 * nr + 0] = (double)pj;
+        rot[4 * nr + 1] = (double)j;
+        rot[4 * nr + 2] = c;
+        rot[4 * nr + 3] = s;
+        ++nr;
+        const double a = dk[pj] * c * c + dk[j] * s * s;
+        dk[j] = dk[pj] * s * s + dk[j] * c * c;
+        dk[pj] = a;
+        int q = nd++;
+        while (q > 0 && dk[dfl[q - 1]] > dk[pj]) { dfl[q] = dfl[q - 1]; --q; }
+        dfl[q] = pj;
+        pj = j;
+      } else {
+        keep[K++] = pj;
+        pj = j;
+      }
+    }
+    if (pj >= 0) keep[K++] = pj;
+    B.nrot[blockIdx.x] = nr;
+    B.kcnt[blockIdx.x] = K;
+    B.rho[blockIdx.x] = rho;
+    sK = K;
+  }
+  __syncthreads();
+  const int K = sK;
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    if (t < K) {
+      B.dkk[lo + t] = dk[keep[t]];
+      B.zkk[lo + t] = zk[keep[t]];
+    } else {
+      B.dval[lo + t - K] = dk[dfl[t - K]];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restrict__ descs, double* __restrict__ Z,
+                                                      int ldz, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.y];
+  const int lo = md.lo, ns = md.hi - md.lo;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ns) return;
+  const int nr = B.nrot[blockIdx.y];
+  const int K = B.kcnt[blockIdx.y];
+  const int* src = B.src + lo;
+  const double* rot = B.rot + 4 * (size_t)lo;
+  double* zr = Z + (lo + r);
+  for (int t = 0; t < nr; ++t) {
+    const int pj = (int)rot[4 * t], j = (int)rot[4 * t + 1];
+    const double c = rot[4 * t + 2], s = rot[4 * t + 3];
+    const size_t cp = (size_t)(lo + src[pj]) * ldz, cj = (size_t)(lo + src[j]) * ldz;
+    const double x = zr[cp], y = zr[cj];
+    zr[cp] = c * x + s * y;
+    zr[cj] = c * y - s * x;
+  }
+  double* Q = B.Qtmp + (size_t)lo * ldz;
+  for (int k = 0; k < ns; ++k) {
+    const int sp = (k < K) ? B.keep[lo + k] : B.dfl[lo + k - K];
+    Q[r + (size_t)k * ns] = zr[(size_t)(lo + src[sp]) * ldz];
+  }
+}
+
+__device__ __forceinline__ double secular_f(const double* d, const double* z, int K, int org, double rho, double t) {
+  const double dorg = d[org];
+  double s = 0.0;
+  for (int j = 0; j < K; ++j) {
+    const double zj = z[j];
+    s += zj * zj / ((d[j] - dorg) - t);
+  }
+  return 1.0 + rho * s;
+}
+
+__global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restrict__ descs, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.y];
+  const int lo = md.lo;
+  const int K = B.kcnt[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= K) return;
+  const double rho = B.rho[blockIdx.y];
+  const double* d = B.dkk + lo;
+  const double* z = B.zkk + lo;
+  int org;
+  double a, b;
+  if (i < K - 1) {
+    const double half = 0.5 * (d[i + 1] - d[i]);
+    if (secular_f(d, z, K, i, rho, half) >= 0.0) { org = i; a = 0.0; b = half; }
+    else { org = i + 1; a = -half; b = 0.0; }
+  } else {
+    double zz = 0.0;
+    for (int j = 0; j < K; ++j) zz += z[j] * z[j];
+    org = i; a = 0.0; b = rho * zz;
+  }
+  for (int it = 0; it < 1200; ++it) {
+    const double m = 0.5 * (a + b);
+    if (!(m > a && m < b)) break;
+    if (secular_f(d, z, K, org, rho, m) > 0.0) b = m; else a = m;
+  }
+  B.org[lo + i] = org;
+  B.tau[lo + i] = (org == i) ? b : a;
+}
+
+__global__ void __launch_bounds__(256) k_merge_zhat(const MergeDesc* __restrict__ descs, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.y];
+  const int lo = md.lo;
+  const int K = B.kcnt[blockIdx.y];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= K) return;
+  const double rho = B.rho[blockIdx.y];
+  const double* d = B.dkk + lo;
+  const int* org = B.org + lo;
+  const double* tau = B.tau + lo;
+  const double dj = d[j];
+  double prod = ((d[org[j]] - dj) + tau[j]) / rho;
+  for (int i = 0; i < K; ++i) {
+    if (i == j) continue;
+    prod *= ((d[org[i]] - dj) + tau[i]) / (d[i] - dj);
+  }
+  B.zhat[lo + j] = copysign(sqrt(fmax(prod, 0.0)), B.zkk[lo + j]);
+}
+
+__global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restrict__ descs, int ldz, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.y];
+  const int lo = md.lo, ns = md.hi - md.lo;
+  const int K = B.kcnt[blockIdx.y];
+  const int i = blockIdx.x;
+  if (i >= K) return;
+  const double* d = B.dkk + lo;
+  const double* zh = B.zhat + lo;
+  const int o = B.org[lo + i];
+  const double t = B.tau[lo + i];
+  const double dorg = d[o];
+  double* Ucol = B.U + (size_t)lo * ldz + (size_t)i * ns;
+  double ss = 0.0;
+  for (int j = threadIdx.x; j < K; j += 256) {
+    const double v = zh[j] / ((d[j] - dorg) - t);
+    Ucol[j] = v;
+    ss += v * v;
+  }
+  __shared__ double sh[4];
+  ss = wave_sum_d(ss);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  const double inv = 1.0 / sqrt(sh[0] + sh[1] + sh[2] + sh[3]);
+  for (int j = threadIdx.x; j < K; j += 256) Ucol[j] *= inv;
+}
+
+__global__ void __launch_bounds__(256) k_merge_order(const MergeDesc* __restrict__ descs, double* __restrict__ lam,
+                                                     MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.x];
+  const int lo = md.lo, ns = md.hi - md.lo;
+  const int K = B.kcnt[blockIdx.x];
+  const int nd = ns - K;
+  // roots are written to lam only after every thread has read the deflated list
+  double* dv = B.dval + lo;
+  const double* d = B.dkk + lo;
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    if (t < K) {
+      const double v = d[B.org[lo + t]] + B.tau[lo + t];
+      B.posK[lo + t] = t + count_less(dv, nd, v);
+      B.zhat[lo + t] = v;  // zhat no longer needed: keep the root value
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    if (t < nd) {
+      const double v = dv[t];
+      // number of roots <= v: roots are sorted ascending
+      int a = 0, b = K;
+      while (a < b) {
+        const int m = (a + b) >> 1;
+        if (B.zhat[lo + m] <= v) a = m + 1; else b = m;
+      }
+      B.posD[lo + t] = t + a;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ns; t += 256) {
+    if (t < K) lam[lo + B.posK[lo + t]] = B.zhat[lo + t];
+    else lam[lo + B.posD[lo + t - K]] = dv[t - K];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_merge_copydefl(const MergeDesc* __restrict__ descs, double* __restrict__ Z,
+                                                        int ldz, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.z];
+  const int lo = md.lo, ns = md.hi - md.lo;
+  const int K = B.kcnt[blockIdx.z];
+  const int t = blockIdx.y;
+  if (t >= ns - K) return;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ns) return;
+  const double* Q = B.Qtmp + (size_t)lo * ldz;
+  Z[(lo + r) + (size_t)(lo + B.posD[lo + t]) * ldz] = Q[r + (size_t)(K + t) * ns];
+}
+
+__global__ void k_dc_unscale(int n, double* __restrict__ lam, const double* __restrict__ scal, double* __restrict__ w) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) w[i] = lam[i] * scal[0];
+}
+
+// --------------------------------------------------------------------------
+// eigenvalues only: Sturm-count bisection, one eigenvalue per thread, on the
+// scaled tridiagonal.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++cnt;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(256) k_bisect(int n, const double* __restrict__ d, const double* __restrict__ e2,
+                                                double gl, double gu, double pivmin, const double* __restrict__ scal,
+                                                double* __restrict__ w) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  double a = gl, b = gu;   // count(a) <= k < count(b)
+  for (int it = 0; it < 200; ++it) {
+    const double m = 0.5 * (a + b);
+    if (!(m > a && m < b)) break;
+    if (b - a <= 2.0 * DEPS * fmax(fabs(a), fabs(b))) break;
+    if (sturm_count(n, d, e2, m, pivmin) > k) b = m; else a = m;
+  }
+  w[k] = 0.5 * (a + b) * scal[0];
+}
+
+__global__ void k_square(int n, const double* __restrict__ es, double* __restrict__ e2) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n - 1) e2[i] = es[i] * es[i];
+}
+
+// --------------------------------------------------------------------------
+static void build_tree(int lo, int hi, int depth, std::vector<int>& leaves, std::vector<std::vector<MergeDesc>>& lv,
+                       std::vector<int>& splits) {
+  if (hi - lo <= LEAF) {
+    leaves.push_back(lo);
+    leaves.push_back(hi);
+    return;
+  }
+  const int mid = lo + (hi - lo) / 2;
+  if ((int)lv.size() <= depth) lv.resize(depth + 1);
+  lv[depth].push_back(MergeDesc{lo, mid, hi});
+  splits.push_back(mid);
+  build_tree(lo, mid, depth + 1, leaves, lv, splits);
+  build_tree(mid, hi, depth + 1, leaves, lv, splits);
+}
+
+size_t stedc_int_slots(int n) {
+  // leaves (2 per leaf) + splits + merge descriptors (3 ints each)
+  const int nleaf = cdiv(n, LEAF) * 2 + 2;
+  return (size_t)2 * nleaf + nleaf + 3 * (size_t)nleaf + 64;
+}
+
+int stedc(int n, const double* d, const double* e, double* w, const StedcWork& sw, bool vectors, hipStream_t s) {
+  if (n <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(1024), 0, s, n, d, e, sw.lam, sw.es, sw.scal);
+  if (!vectors) {
+    // Gershgorin interval of the scaled matrix
+    hipLaunchKernelGGL(k_square, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.es, sw.e2);
+    const double pivmin = DBL_MIN * 4.0;
+    hipLaunchKernelGGL(k_bisect, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.lam, sw.e2, -3.0 - 1e-12, 3.0 + 1e-12,
+                       pivmin, sw.scal, w);
+    return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+  }
+  std::vector<int> leaves, splits;
+  std::vector<std::vector<MergeDesc>> levels;
+  build_tree(0, n, 0, leaves, levels, splits);
+  const int nleaf = (int)leaves.size() / 2;
+  // descriptors go to the device through the integer scratch
+  int* ileaf = sw.ints;
+  int* isplit = ileaf + leaves.size();
+  MergeDesc* idesc = reinterpret_cast<MergeDesc*>(isplit + splits.size() + (splits.size() & 1));
+  size_t ndesc = 0;
+  for (auto& L : levels) ndesc += L.size();
+  std::vector<MergeDesc> flat;
+  flat.reserve(ndesc);
+  std::vector<size_t> level_off;
+  for (int dpt = (int)levels.size() - 1; dpt >= 0; --dpt) {
+    level_off.push_back(flat.size());
+    for (auto& md : levels[dpt]) flat.push_back(md);
+  }
+  std::vector<int> host(leaves.size() + splits.size() + (splits.size() & 1) + 3 * flat.size());
+  size_t o = 0;
+  for (int v : leaves) host[o++] = v;
+  for (int v : splits) host[o++] = v;
+  if (splits.size() & 1) host[o++] = 0;
+  for (auto& md : flat) { host[o++] = md.lo; host[o++] = md.mid; host[o++] = md.hi; }
+  if (host.size() > sw.int_slots) return ZQI_ERR_ARG;
+  ZQ_HIP_CHECK(hipMemcpyAsync(sw.ints, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  ZQ_HIP_CHECK(hipMemsetAsync(sw.Z, 0, (size_t)n * n * sizeof(double), s));
+  if (!splits.empty())
+    hipLaunchKernelGGL(k_dc_tear, dim3(cdiv((int)splits.size(), 256)), dim3(256), 0, s, (int)splits.size(), isplit,
+                       sw.es, sw.lam);
+  hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, n, ileaf, sw.lam, sw.es, sw.Z, n, sw.info);
+  MergeBufs B{sw.dk, sw.zk, sw.src, sw.keep, sw.dfl, sw.rot, sw.nrot, sw.kcnt, sw.dkk, sw.zkk, sw.dval, sw.rho,
+              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U};
+  for (size_t li = 0; li < level_off.size(); ++li) {
+    const size_t off = level_off[li];
+    const size_t cnt = (li + 1 < level_off.size() ? level_off[li + 1] : flat.size()) - off;
+    const MergeDesc* dd = idesc + off;
+    int nsmax = 0;
+    for (size_t q = 0; q < cnt; ++q) nsmax = std::max(nsmax, flat[off + q].hi - flat[off + q].lo);
+    const int nm = (int)cnt;
+    hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, n, sw.lam, sw.es, sw.Z, n, B);
+    hipLaunchKernelGGL(k_merge_rotgat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, n, B);
+    hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, B);
+    hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, B);
+    hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, n, B);
+    hipLaunchKernelGGL(k_merge_order, dim3(nm), dim3(256), 0, s, dd, sw.lam, B);
+    for (int q = 0; q < nm; ++q) {
+      const MergeDesc& md = flat[off + q];
+      const int ns = md.hi - md.lo;
+      int rc = dgemm_nn(ns, 0, sw.kcnt + q, ns, sw.Qtmp + (size_t)md.lo * n, ns, sw.U + (size_t)md.lo * n, ns,
+                        sw.Z + md.lo + (size_t)md.lo * n, n, sw.posK + md.lo, s);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_merge_copydefl, dim3(cdiv(nsmax, 256), nsmax, nm), dim3(256), 0, s, dd, sw.Z, n, B);
+  }
+  hipLaunchKernelGGL(k_dc_unscale, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.lam, sw.scal, w);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+}  // namespace zq
