@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark of the quaternion-Hermitian eigensolver on one MI355X.
+
+Workload: one solve of 2n = 8192 (complex128, A Hermitian / B antisymmetric
+with U(-1,1) entries, eigenvalues + Kramers-paired eigenvectors).  A step is
+one call of the device entry point zq_zquatev_device on a matrix already in
+HBM (the left half is restored from a pristine device copy first: an O(n^2)
+device copy inside the step, < 0.2% of its time).
+
+Prints one JSON line: FP64 GFLOP/s on the algorithmic count (160/3) n^3,
+the roofline of the dominant kernel class measured with HIP events on the
+library's stream, and a CPU baseline (the reference compiled from its own
+sources, timed on this host's cores on a bounded sample).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 GFLOP/s at 2n=8192; batched solves/s + p50/p99 latency vs plain loop"
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n2", type=int, default=8192)
+    ap.add_argument("--nb", type=int, default=0, help="panel width (0: tuned table)")
+    ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only warmup + timed steps (for rocprofv3), no instrumentation pass or baseline")
+    ap.add_argument("--cpu-child", nargs=2, type=int, metavar=("N2", "THREADS"), help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def cpu_child(n2, threads):
+    """Runs in a subprocess with MKL_NUM_THREADS set: one reference solve."""
+    import numpy as np
+    from zquatev_amd import gen
+    n = n2 // 2
+    D = gen.uniform_case(n, 1001)
+    kind = "reference"
+    try:
+        from oracle import ref
+        if not ref.available():
+            raise ImportError("oracle/_ref not built")
+        ref.zquatev(gen.uniform_case(8, 1))      # load MKL outside the timed call
+        t0 = time.perf_counter()
+        info, X, eig = ref.zquatev(D)
+        t = time.perf_counter() - t0
+        assert info == 0
+    except (ImportError, OSError):
+        from oracle import zq_oracle
+        kind = "port"
+        t0 = time.perf_counter()
+        info, X, eig = zq_oracle.zquatev(D)
+        t = time.perf_counter() - t0
+    print(json.dumps({"seconds": t, "kind": kind, "n2": n2, "threads": threads,
+                      "eig0": float(np.asarray(eig)[0])}))
+
+
+def cpu_baseline(n2):
+    threads = max(1, min(16, os.cpu_count() or 1))
+    env = dict(os.environ)
+    env["MKL_NUM_THREADS"] = str(threads)
+    env["OMP_NUM_THREADS"] = str(threads)
+    env["HIP_VISIBLE_DEVICES"] = ""
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", str(n2), str(threads)],
+                         env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        return {"value": None, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+                "sample": f"failed: {out.stderr[-300:]}"}
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    from zquatev_amd import gen
+    n = n2 // 2
+    return {"value": round(gen.flops_vectors(n) / r["seconds"] / 1e9, 3), "unit": "GFLOP/s",
+            "cores": r["threads"], "kind": r["kind"],
+            "sample": (f"one ts::zquatev call (eigenvalues+eigenvectors) at 2n={n2}, same U(-1,1) generator, "
+                       f"{r['threads']} MKL threads, {r['seconds']:.2f} s; GFLOP/s on (160/3) n^3")}
+
+
+def main():
+    args = parse()
+    if args.cpu_child:
+        cpu_child(*args.cpu_child)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the solver is single-device; multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import zquatev_amd as zq
+    from zquatev_amd import gen
+
+    n2 = args.n2
+    n = n2 // 2
+    D = gen.uniform_case(n, args.seed)
+    dev = torch.device("cuda:0")
+    # row-major D^T == column-major D
+    Dt = torch.from_numpy(np.ascontiguousarray(D.T)).to(dev)
+    pristine = Dt[:n, :].clone()          # left half (first n columns of D)
+    eig = torch.zeros(n, dtype=torch.float64, device=dev)
+    ctx = zq.Context(0, panel_width=args.nb)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    nb = args.nb if args.nb else zq.tuned_panel_width(n)
+
+    def step():
+        Dt[:n, :].copy_(pristine)
+        rc = ctx.zquatev_device(n2, Dt.data_ptr(), n2, eig.data_ptr(), "V")
+        if rc != 0:
+            raise RuntimeError(f"zq_zquatev_device failed: {zq.error_string(rc)} ({rc})")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    phases = ctx.last_timings()
+    flops = gen.flops_vectors(n)
+    value = flops / (ms_per_step / 1000.0) / 1e9
+
+    if args.profile_run:
+        print(json.dumps({"ms_per_step": ms_per_step, "gflops": value, "phases_ms": phases}))
+        return 0
+
+    check = None
+    if not args.no_check:
+        # size-independent checks on the device result: trace identity and the
+        # residual / orthonormality of 64 eigenpairs against H rebuilt from the input
+        A = pristine[:, :n].T          # (n x n) = D[:n, :n]
+        B = pristine[:, n:].T          # D[n:, :n]
+        H = torch.cat([torch.cat([A, -B.conj()], 1), torch.cat([B, A.conj()], 1)], 0)
+        X = Dt.T                       # column-major view of the result
+        cols = torch.arange(0, n2, max(1, n2 // 64), device=dev)
+        Xs = X[:, cols].contiguous()
+        lam = torch.cat([eig, eig])[cols]
+        g = torch.Generator(device=dev).manual_seed(7)
+        v = torch.randn(n2, dtype=torch.float64, device=dev, generator=g).to(H.dtype)
+        for _ in range(30):            # power iteration: a lower bound of ||H||_2
+            v = H @ v
+            v = v / torch.linalg.vector_norm(v)
+        hn = torch.linalg.vector_norm(H @ v).item()
+        res = (torch.linalg.norm(H @ Xs - Xs * lam[None, :]) / (n * hn)).item()
+        orth = (Xs.conj().T @ Xs - torch.eye(len(cols), dtype=Xs.dtype, device=dev)).abs().max().item()
+        trace = abs(2.0 * eig.sum().item() - torch.diagonal(H).real.sum().item()) / (n2 * hn)
+        check = {"residual_64": res, "orth_64": orth, "trace_rel": trace,
+                 "ok": bool(res <= 1e-13 and orth <= 1e-13 and trace <= 1e-13)}
+        del H
+
+    # instrumented pass: per-kernel-class HIP event times on the library's stream
+    ctx.set_instrument(True)
+    step()
+    torch.cuda.synchronize()
+    kernels = {}
+    for cls in ctx.KERNEL_CLASSES:
+        t, k, u = ctx.instrument_report(cls)
+        kernels[cls] = {"launches": k, "total_ms": round(t, 3), "avg_ms": (t / k if k else None), "units": u}
+    ctx.set_instrument(False)
+    dom = max(("symv", "trail", "backtr"), key=lambda c: kernels[c]["total_ms"])
+    kd = kernels[dom]
+    if dom == "symv":
+        ach = kd["units"] / (kd["total_ms"] / 1000.0) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_symv (panel Hermitian matvec)"}
+    else:
+        ach = kd["units"] / (kd["total_ms"] / 1000.0) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_MFMA_PEAK_TFS, 4), "traffic": None,
+                "kernel": "k_qgemm (%s)" % ("trailing rank-2nb update" if dom == "trail" else "back-transform")}
+    roof["avg_launch_ms"] = kd["avg_ms"]
+    roof["launches"] = kd["launches"]
+    for c in kernels:
+        v = kernels[c]
+        if v["launches"] and v["units"]:
+            if c == "symv":
+                v["GB_per_s"] = round(v["units"] / (v["total_ms"] / 1000.0) / 1e9, 1)
+            else:
+                v["TFLOP_per_s"] = round(v["units"] / (v["total_ms"] / 1000.0) / 1e12, 2)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_n2)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single solve, 2n=8192 complex128, A Hermitian / B antisymmetric U(-1,1), "
+                               "eigenvalues + Kramers-paired eigenvectors",
+                   "n2": n2, "panel_width": nb, "seed": args.seed, "generator": "splitmix64",
+                   "flop_count": "(160/3) n^3, n = 2n/2"},
+        "event_ms_per_step": round(ev_ms, 3),
+        "phases_ms": {k: round(v, 3) for k, v in phases.items()},
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "check": check,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -44,9 +44,12 @@ const char* zq_error_string(int code);
 int zq_create(zq_context** ctx, int device);
 int zq_destroy(zq_context* ctx);
 
-/* Use the caller's HIP stream (a hipStream_t passed as void*; NULL restores
- * the context's own stream).  The device entry points are asynchronous on it. */
+/* Use the caller's HIP stream (a hipStream_t passed as void*; NULL is the HIP
+ * null stream).  The device entry points are asynchronous on it. */
 int zq_set_stream(zq_context* ctx, void* stream);
+
+/* Go back to the stream the context created (the default after zq_create). */
+int zq_use_own_stream(zq_context* ctx);
 
 /* Panel width (quaternion columns per panel, 1..128).  0 selects the tuned
  * table.  The reference hard-codes nb = 20 (zquatev.cc:64). */
@@ -92,6 +95,15 @@ int zq_tridiag_eig(zq_context* ctx, int n, const double* dd, const double* de, d
  * ms[0] reduction, ms[1] tridiagonal eigensolve, ms[2] back-transform,
  * ms[3] layout in/out, ms[4] total.  Returns the number of values written. */
 int zq_last_timings(zq_context* ctx, double* ms, int count);
+
+/* Kernel-class timing for measurement harnesses: with instrumentation on,
+ * each launch of an instrumented kernel class is bracketed by HIP events on
+ * the context's stream.  Classes: 0 panel Hermitian matvec (units: bytes of
+ * the trailing lower triangle read), 1 trailing rank-2nb update (units:
+ * flops), 2 back-transform products (flops), 3 divide & conquer merge GEMMs.
+ * zq_instrument_report sums the last solve's launches of one class. */
+int zq_set_instrument(zq_context* ctx, int on);
+int zq_instrument_report(zq_context* ctx, int cls, double* total_ms, int* count, double* units);
 
 /* Self-test of the MFMA register layout the kernels rely on: out (64 x 4
  * doubles, device) = raw accumulator of one v_mfma_f64_16x16x4_f64 with

@@ -27,6 +27,7 @@ _SIGNATURES = {
     "zq_create": (_c_int, [ctypes.POINTER(_vp), _c_int]),
     "zq_destroy": (_c_int, [_vp]),
     "zq_set_stream": (_c_int, [_vp, _vp]),
+    "zq_use_own_stream": (_c_int, [_vp]),
     "zq_set_panel_width": (_c_int, [_vp, _c_int]),
     "zq_tuned_panel_width": (_c_int, [_c_int]),
     "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
@@ -37,6 +38,9 @@ _SIGNATURES = {
     "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
     "zq_selftest_mfma": (_c_int, [_vp, _vp, _vp, _vp]),
     "zq_synchronize": (_c_int, [_vp]),
+    "zq_set_instrument": (_c_int, [_vp, _c_int]),
+    "zq_instrument_report": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_c_int),
+                                      ctypes.POINTER(ctypes.c_double)]),
 }
 
 _LIB = None
@@ -163,7 +167,12 @@ class Context:
         self.close()
 
     def set_stream(self, stream_ptr):
+        """Run on the given HIP stream handle (0 = the null stream, which is
+        torch's default stream)."""
         _check(_fn("zq_set_stream")(self._h, _vp(stream_ptr or 0)), "zq_set_stream")
+
+    def use_own_stream(self):
+        _check(_fn("zq_use_own_stream")(self._h), "zq_use_own_stream")
 
     def set_panel_width(self, nb):
         _check(_fn("zq_set_panel_width")(self._h, int(nb)), "zq_set_panel_width")
@@ -192,6 +201,22 @@ class Context:
 
     def selftest_mfma_device(self, a_ptr, b_ptr, out_ptr):
         return _fn("zq_selftest_mfma")(self._h, _vp(a_ptr), _vp(b_ptr), _vp(out_ptr))
+
+    def set_instrument(self, on=True):
+        _check(_fn("zq_set_instrument")(self._h, 1 if on else 0), "zq_set_instrument")
+
+    KERNEL_CLASSES = ("symv", "trail", "backtr", "merge")
+
+    def instrument_report(self, cls):
+        """(total_ms, launches, algorithmic units) of one kernel class in the last solve."""
+        if isinstance(cls, str):
+            cls = self.KERNEL_CLASSES.index(cls)
+        t = ctypes.c_double()
+        k = _c_int()
+        u = ctypes.c_double()
+        _check(_fn("zq_instrument_report")(self._h, cls, ctypes.byref(t), ctypes.byref(k), ctypes.byref(u)),
+               "zq_instrument_report")
+        return t.value, k.value, u.value
 
     def last_timings(self):
         """Device phase times of the last solve (ms): reduce, tridiagonal, back-transform, layout, total."""

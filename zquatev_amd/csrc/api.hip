@@ -15,6 +15,20 @@
 #include <complex>
 #include <cstring>
 #include <mutex>
+#include <vector>
+
+namespace zq {
+// event-pair records of the instrumented launches of the current solve
+struct ProfState {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> cls;
+  std::vector<double> units;
+  size_t used = 0;
+};
+static thread_local ProfState* g_prof = nullptr;
+
+}  // namespace zq
 
 struct zq_context {
   int device = 0;
@@ -30,7 +44,32 @@ struct zq_context {
   hipEvent_t ev[6] = {};
   bool have_times = false;
   int* info_dev = nullptr;  // leaf-solver status of the last vectors solve
+  zq::ProfState prof;
 };
+
+
+namespace zq {
+ProfScope::ProfScope(int c, hipStream_t st, double u) : cls(c), s(st), units(u), slot(-1) {
+  ProfState* p = g_prof;
+  if (!p || !p->on) return;
+  if (p->pool.size() < 2 * (p->used + 1)) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      p->pool.push_back(e);
+    }
+  }
+  slot = (int)p->used++;
+  if (p->cls.size() < p->used) { p->cls.resize(p->used); p->units.resize(p->used); }
+  p->cls[slot] = c;
+  p->units[slot] = u;
+  (void)hipEventRecord(p->pool[2 * slot], s);
+}
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  (void)hipEventRecord(g_prof->pool[2 * slot + 1], s);
+}
+}  // namespace zq
 
 namespace {
 
@@ -162,6 +201,9 @@ int with_device(zq_context* ctx) {
 // The device solve: dD (ld2 complex) -> dD, deig.
 int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool vectors) {
   hipStream_t s = ctx->stream;
+  ctx->prof.used = 0;
+  zq::g_prof = &ctx->prof;
+  struct Reset { ~Reset() { zq::g_prof = nullptr; } } reset_prof;
   const int nb = panel_width(ctx, n);
   Plan p;
   int rc = make_plan(ctx, n, nb, vectors, &p);
@@ -255,6 +297,7 @@ int zq_destroy(zq_context* c) {
   if (c->deig) (void)hipFree(c->deig);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->prof.pool) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return ZQ_OK;
@@ -262,7 +305,13 @@ int zq_destroy(zq_context* c) {
 
 int zq_set_stream(zq_context* c, void* stream) {
   if (!c) return -1;
-  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  c->stream = static_cast<hipStream_t>(stream);
+  return ZQ_OK;
+}
+
+int zq_use_own_stream(zq_context* c) {
+  if (!c) return -1;
+  c->stream = c->own_stream;
   return ZQ_OK;
 }
 
@@ -368,6 +417,31 @@ int zq_last_timings(zq_context* c, double* ms, int count) {
   int k = count < 5 ? count : 5;
   for (int i = 0; i < k; ++i) ms[i] = t[i];
   return k;
+}
+
+int zq_set_instrument(zq_context* c, int on) {
+  if (!c) return ZQ_ERR_ARG;
+  c->prof.on = on != 0;
+  return ZQ_OK;
+}
+
+int zq_instrument_report(zq_context* c, int cls, double* total_ms, int* count, double* units) {
+  if (!c || cls < 0 || cls >= zq::PROF_NCLASS) return ZQ_ERR_ARG;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
+  double t = 0.0, u = 0.0;
+  int k = 0;
+  for (size_t i = 0; i < c->prof.used; ++i) {
+    if (c->prof.cls[i] != cls) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->prof.pool[2 * i], c->prof.pool[2 * i + 1]) != hipSuccess) return ZQ_ERR_HIP;
+    t += ms;
+    u += c->prof.units[i];
+    ++k;
+  }
+  if (total_ms) *total_ms = t;
+  if (count) *count = k;
+  if (units) *units = u;
+  return ZQ_OK;
 }
 
 int zq_selftest_mfma(zq_context* c, const double* dA, const double* dB, double* dOut) {

@@ -97,7 +97,11 @@ int backtransform(int n, int nb, const quat* M, int ld, const quat* Tf, const qu
     QOperand Vc{M + r0 + (size_t)p * ld, M + r0 + (size_t)p * ld, ld, ld, INT_MAX, true, -1};
     const int nz = splitk_count(mr, w.nsplit, nullptr);
     if ((size_t)nz * wdt * n > w.ypart_slots) return ZQI_ERR_ARG;
-    int rc = qgemm_cn_splitk(wdt, n, mr, Vc, X + r0, ldx, w.Ypart, w.nsplit, s);
+    int rc;
+    {
+      ProfScope ps(PROF_BACKTR, s, 32.0 * (double)mr * wdt * n);
+      rc = qgemm_cn_splitk(wdt, n, mr, Vc, X + r0, ldx, w.Ypart, w.nsplit, s);
+    }
     if (rc) return rc;
     const int total = wdt * n;
     hipLaunchKernelGGL(k_sum_parts, dim3(cdiv(total, 256)), dim3(256), 0, s, total, nz, w.Ypart, w.Y);
@@ -108,7 +112,10 @@ int backtransform(int n, int nb, const quat* M, int ld, const quat* Tf, const qu
     if (rc) return rc;
     QOperand Vn{M + r0 + (size_t)p * ld, M + r0 + (size_t)p * ld, ld, ld, INT_MAX, false, -1};
     QOperand Y2o{w.Y2, w.Y2, wdt, wdt, INT_MAX, false, INT_MIN / 2};
-    rc = qgemm(mr, n, wdt, -1.0, Vn, Y2o, 1.0, X + r0, ldx, false, s);
+    {
+      ProfScope ps(PROF_BACKTR, s, 32.0 * (double)mr * wdt * n);
+      rc = qgemm(mr, n, wdt, -1.0, Vn, Y2o, 1.0, X + r0, ldx, false, s);
+    }
     if (rc) return rc;
   }
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;

@@ -301,8 +301,12 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       const int nt = cdiv(sz, TB);
       const int ntiles = nt * (nt + 1) / 2;
       const int nch = (j > 0) ? cdiv(sz, TSR) : 0;
-      hipLaunchKernelGGL(k_symv, dim3(ntiles + nch), dim3(256), 0, s, M, ld, n, p, c, j, w.W, ldw, w.scal, w.P,
-                         n, w.ucur, w.TS, nb, ntiles);
+      {
+        // algorithmic bytes: one read of the trailing lower triangle
+        ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
+        hipLaunchKernelGGL(k_symv, dim3(ntiles + nch), dim3(256), 0, s, M, ld, n, p, c, j, w.W, ldw, w.scal, w.P,
+                           n, w.ucur, w.TS, nb, ntiles);
+      }
       const int nb2 = cdiv(sz, RB);
       hipLaunchKernelGGL(k_wreduce, dim3(nb2), dim3(RB), 0, s, M, ld, n, p, c, j, w.W, ldw, w.P, n, nt, w.TS, nch,
                          nb, w.ucur, w.KP, Tf, w.scal);
@@ -314,6 +318,8 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
     if (mt > 0) {
       QOperand A{M + r0 + (size_t)p * ld, w.W + r0, ld, ldw, wdt, false, INT_MIN / 2};
       QOperand B{w.W + r0, M + r0 + (size_t)p * ld, ldw, ld, wdt, true, INT_MIN / 2};
+      // algorithmic flops: lower triangle x rank 2*wdt, 32 flops per quaternion multiply-add
+      ProfScope ps(PROF_TRAIL, s, 32.0 * 0.5 * (double)mt * (mt + 1) * 2.0 * wdt);
       int rc = qgemm(mt, mt, 2 * wdt, -1.0, A, B, 1.0, M + r0 + (size_t)r0 * ld, ld, true, s);
       if (rc) return rc;
     }

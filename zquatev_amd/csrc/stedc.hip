@@ -598,6 +598,7 @@ int stedc(int n, const double* d, const double* e, double* w, const StedcWork& s
     for (int q = 0; q < nm; ++q) {
       const MergeDesc& md = flat[off + q];
       const int ns = md.hi - md.lo;
+      ProfScope ps(PROF_MERGE, s, 0.0);
       int rc = dgemm_nn(ns, 0, sw.kcnt + q, ns, sw.Qtmp + (size_t)md.lo * n, ns, sw.U + (size_t)md.lo * n, ns,
                         sw.Z + md.lo + (size_t)md.lo * n, n, sw.posK + md.lo, s);
       if (rc) return rc;

@@ -112,6 +112,19 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 
 namespace zq {
 
+// Optional per-kernel-class timing with HIP events (enabled per context).
+// Classes: symv (panel matvec), trail (trailing MFMA update), backtr (MFMA
+// back-transform products), merge (D&C merge GEMM), other.
+enum ProfClass { PROF_SYMV = 0, PROF_TRAIL = 1, PROF_BACKTR = 2, PROF_MERGE = 3, PROF_NCLASS = 4 };
+struct ProfScope {
+  int cls;
+  hipStream_t s;
+  double units;
+  int slot;
+  ProfScope(int cls, hipStream_t s, double units);
+  ~ProfScope();
+};
+
 // reduce.hip: quaternion Householder tridiagonalisation of the lower triangle
 // of M (n x n, ld).  Writes d (n), e (n-1 quaternions: the subdiagonal),
 // Tf (per panel nb x nb compact-WY factors) and leaves u_c in M[c+1:, c].
