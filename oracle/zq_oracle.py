@@ -41,6 +41,17 @@ def _dlapy3(x, y, z):
     return w * np.sqrt((x / w) ** 2 + (y / w) ** 2 + (z / w) ** 2)
 
 
+def _dznrm2(x):
+    """2-norm without underflow or overflow (LAPACK dznrm2 semantics)."""
+    if x.size == 0:
+        return 0.0
+    scale = float(np.max(np.abs(np.concatenate([x.real, x.imag]))))
+    if scale == 0.0:
+        return 0.0
+    y = x / scale
+    return scale * float(np.sqrt(np.sum(y.real * y.real + y.imag * y.imag)))
+
+
 def zlarfg(alpha, x):
     """Elementary reflector H = I - tau [1; v][1; v]^H with H^H [alpha; x] = [beta; 0],
     beta real (LAPACK zlarfg).  Returns (beta, v, tau); x is not modified.
@@ -50,7 +61,7 @@ def zlarfg(alpha, x):
     alpha = complex(alpha)
     if x.size == 0:
         return alpha, x.copy(), 0.0 + 0.0j
-    xnorm = float(np.linalg.norm(x))
+    xnorm = _dznrm2(x)
     alphr, alphi = alpha.real, alpha.imag
     if xnorm == 0.0 and alphi == 0.0:
         return alpha, x.copy(), 0.0 + 0.0j
@@ -68,7 +79,7 @@ def zlarfg(alpha, x):
             alphr *= rsafmn
             if not (abs(beta) < safmin and knt < 20):
                 break
-        xnorm = float(np.linalg.norm(xs))
+        xnorm = _dznrm2(xs)
         alpha = complex(alphr, alphi)
         beta = -np.copysign(_dlapy3(alphr, alphi, xnorm), alphr)
     tau = complex((beta - alphr) / beta, -alphi / beta)

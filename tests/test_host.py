@@ -1,0 +1,84 @@
+"""CPU tests of the C ABI boundary: the library loads, exports every symbol
+declared in include/*.h, and rejects bad arguments before touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import zquatev_amd as zq
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in ("zq.h",):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(zq_\w+)\s*\(", src, flags=re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = zq.lib()
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", zq.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (\w+)", out))
+    missing = declared - exported
+    assert not missing, missing
+    for name in declared:
+        getattr(lib, name)
+    # the C++ drop-in ts::zquatev (include/zquatev.h) with the reference's mangled name
+    assert "_ZN2ts7zquatevEiPSt7complexIdEiPd" in exported
+
+
+def test_python_binding_covers_abi():
+    assert _declared_functions() <= set(zq._SIGNATURES)
+
+
+def test_argument_validation_without_device():
+    D = np.zeros((4, 4), dtype=np.complex128, order="F")
+    eig = np.zeros(2)
+    f = zq.lib().zq_zquatev
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert f(3, D.ctypes.data, 4, eig.ctypes.data) == -1       # odd n2
+    assert f(-2, D.ctypes.data, 4, eig.ctypes.data) == -1
+    assert f(4, None, 4, eig.ctypes.data) == -2
+    assert f(4, D.ctypes.data, 2, eig.ctypes.data) == -3       # nld2 < n2
+    assert f(4, D.ctypes.data, 4, None) == -4
+    assert f(0, None, 1, None) == 0                            # empty problem
+    ts = zq.lib()._ZN2ts7zquatevEiPSt7complexIdEiPd
+    ts.restype = ctypes.c_int
+    ts.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert ts(5, D.ctypes.data, 4, eig.ctypes.data) == -1
+    info, _ = zq.zquatev(np.zeros((4, 4), dtype=np.complex128, order="F"), n2=3)
+    assert info == -1
+
+
+def test_error_strings_and_tuning_table():
+    assert zq.error_string(0) == "success"
+    assert "odd" in zq.error_string(-1)
+    assert "memory" in zq.error_string(zq.ERR_NOMEM)
+    assert "converge" in zq.error_string(3)
+    for n in (1, 32, 256, 4096, 16384):
+        assert 1 <= zq.tuned_panel_width(n) <= 128
+    assert zq.version() >= 100
+
+
+def test_input_type_checks():
+    with pytest.raises(TypeError):
+        zq.zquatev(np.zeros((4, 4)))
+    with pytest.raises(ValueError):
+        zq.zquatev(np.zeros((4, 4), dtype=np.complex128, order="C")[:, :3])
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

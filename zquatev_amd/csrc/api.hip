@@ -355,6 +355,8 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
 }
 
 int zq_zquatev(int n2, double* D, int nld2, double* eig) {
+  int rc = check_args(n2, D, nld2, eig);
+  if (rc || n2 == 0) return rc;
   // one solve at a time on the shared context
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_ctx) {
