@@ -342,62 +342,122 @@ __global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restric
   }
 }
 
-__device__ __forceinline__ double secular_f(const double* d, const double* z, int K, int org, double rho, double t) {
-  const double dorg = d[org];
-  double s = 0.0;
-  for (int j = 0; j < K; ++j) {
-    const double zj = z[j];
-    s += zj * zj / ((d[j] - dorg) - t);
+// One root of f(t) = 1 + rho sum_j z_j^2 / (delta_j - t) per wavefront, the 64
+// lanes splitting the sum.  Origin: the nearer pole (chosen by the sign of f
+// at the midpoint of the interval), so t is small and the differences
+// delta_j - t are formed without cancellation.  Iteration: Newton on the
+// pole-free G(t) = (t - p_lo)(p_hi - t) f(t), safeguarded by a bisection
+// bracket, until the step falls below a few ulps of t.
+struct SecSums {
+  double r, rp;   // rest = rho sum over the non-pole terms, and its derivative
+};
+
+__device__ __forceinline__ SecSums sec_rest(const double* d, const double* z, int K, double dorg, double rho, double t,
+                                            int plo, int phi) {
+  const int lane = threadIdx.x & 63;
+  double r = 0.0, rp = 0.0;
+  for (int jj = lane; jj < K; jj += 64) {
+    if (jj == plo || jj == phi) continue;
+    const double zj = z[jj];
+    const double inv = 1.0 / ((d[jj] - dorg) - t);
+    const double q = zj * zj * inv;
+    r += q;
+    rp += q * inv;
   }
-  return 1.0 + rho * s;
+  r = wave_sum_d(r);
+  rp = wave_sum_d(rp);
+  return SecSums{rho * r, rho * rp};
 }
 
 __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restrict__ descs, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.y];
   const int lo = md.lo;
   const int K = B.kcnt[blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= K) return;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= K) return;   // wave-uniform
   const double rho = B.rho[blockIdx.y];
   const double* d = B.dkk + lo;
   const double* z = B.zkk + lo;
-  int org;
-  double a, b;
+  const int lane = threadIdx.x & 63;
+  int org, plo, phi;
+  double a, b, pl, ph;   // bracket (a, b) and pole positions relative to the origin
   if (i < K - 1) {
-    const double half = 0.5 * (d[i + 1] - d[i]);
-    if (secular_f(d, z, K, i, rho, half) >= 0.0) { org = i; a = 0.0; b = half; }
-    else { org = i + 1; a = -half; b = 0.0; }
+    const double gap = d[i + 1] - d[i];
+    const double half = 0.5 * gap;
+    // f at the midpoint with origin d_i
+    double s = 0.0;
+    for (int jj = lane; jj < K; jj += 64) {
+      const double zj = z[jj];
+      s += zj * zj / ((d[jj] - d[i]) - half);
+    }
+    s = wave_sum_d(s);
+    plo = i;
+    phi = i + 1;
+    if (1.0 + rho * s >= 0.0) { org = i; a = 0.0; b = half; pl = 0.0; ph = gap; }
+    else { org = i + 1; a = -half; b = 0.0; pl = -gap; ph = 0.0; }
   } else {
     double zz = 0.0;
-    for (int j = 0; j < K; ++j) zz += z[j] * z[j];
-    org = i; a = 0.0; b = rho * zz;
+    for (int jj = lane; jj < K; jj += 64) zz += z[jj] * z[jj];
+    zz = wave_sum_d(zz);
+    org = i; plo = i; phi = -1;
+    a = 0.0; b = rho * zz; pl = 0.0; ph = 0.0;
   }
-  for (int it = 0; it < 1200; ++it) {
-    const double m = 0.5 * (a + b);
-    if (!(m > a && m < b)) break;
-    if (secular_f(d, z, K, org, rho, m) > 0.0) b = m; else a = m;
+  const double dorg = d[org];
+  const double zl2 = rho * z[plo] * z[plo];
+  const double zh2 = (phi >= 0) ? rho * z[phi] * z[phi] : 0.0;
+  double t = 0.5 * (a + b);
+  for (int it = 0; it < 200; ++it) {
+    const SecSums S = sec_rest(d, z, K, dorg, rho, t, plo, phi);
+    const double one = 1.0 + S.r;
+    double G, Gp;
+    if (phi >= 0) {
+      const double tl = t - pl, th = ph - t;
+      G = tl * th * one - zl2 * th + zh2 * tl;
+      Gp = (th - tl) * one + tl * th * S.rp + zl2 + zh2;
+    } else {
+      const double tl = t - pl;
+      G = tl * one - zl2;
+      Gp = one + tl * S.rp;
+    }
+    if (G == 0.0) { a = b = t; break; }
+    if (G < 0.0) a = t; else b = t;
+    double tn = t - G / Gp;
+    if (!(tn > a && tn < b)) tn = 0.5 * (a + b);
+    const double step = fabs(tn - t);
+    t = tn;
+    if (!(b > a) || step <= 4.0 * DEPS * fabs(t) || !(0.5 * (a + b) > a && 0.5 * (a + b) < b)) break;
   }
-  B.org[lo + i] = org;
-  B.tau[lo + i] = (org == i) ? b : a;
+  // never return the pole itself
+  if (org == i && !(t > 0.0)) t = b > 0.0 ? b : DBL_MIN;
+  if (org != i && !(t < 0.0)) t = a < 0.0 ? a : -DBL_MIN;
+  if (lane == 0) {
+    B.org[lo + i] = org;
+    B.tau[lo + i] = t;
+  }
 }
 
+// Gu-Eisenstat: zhat_j^2 = prod_i (lambda_i - d_j) / (rho prod_{i != j} (d_i - d_j)),
+// one wavefront per j, the lanes splitting the product.
 __global__ void __launch_bounds__(256) k_merge_zhat(const MergeDesc* __restrict__ descs, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.y];
   const int lo = md.lo;
   const int K = B.kcnt[blockIdx.y];
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= K) return;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= K) return;   // wave-uniform
+  const int lane = threadIdx.x & 63;
   const double rho = B.rho[blockIdx.y];
   const double* d = B.dkk + lo;
   const int* org = B.org + lo;
   const double* tau = B.tau + lo;
   const double dj = d[j];
-  double prod = ((d[org[j]] - dj) + tau[j]) / rho;
-  for (int i = 0; i < K; ++i) {
-    if (i == j) continue;
-    prod *= ((d[org[i]] - dj) + tau[i]) / (d[i] - dj);
+  double prod = 1.0;
+  for (int i = lane; i < K; i += 64) {
+    const double num = (d[org[i]] - dj) + tau[i];
+    prod *= (i == j) ? num / rho : num / (d[i] - dj);
   }
-  B.zhat[lo + j] = copysign(sqrt(fmax(prod, 0.0)), B.zkk[lo + j]);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) prod *= __shfl_xor(prod, m, 64);
+  if (lane == 0) B.zhat[lo + j] = copysign(sqrt(fmax(prod, 0.0)), B.zkk[lo + j]);
 }
 
 __global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restrict__ descs, int ldz, MergeBufs B) {
@@ -591,8 +651,8 @@ int stedc(int n, const double* d, const double* e, double* w, const StedcWork& s
     const int nm = (int)cnt;
     hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, n, sw.lam, sw.es, sw.Z, n, B);
     hipLaunchKernelGGL(k_merge_rotgat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, n, B);
-    hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, B);
-    hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, B);
+    hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
+    hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, n, B);
     hipLaunchKernelGGL(k_merge_order, dim3(nm), dim3(256), 0, s, dd, sw.lam, B);
     for (int q = 0; q < nm; ++q) {

@@ -131,12 +131,14 @@ struct ProfScope {
 struct ReduceWork {
   quat* W;        // n x nb (global row index)
   quat* ucur;     // n
-  quat* P;        // symv partials: ceil((n-1)/32) x n
-  quat* TS;       // tall-skinny partials: ceil(n/256) x 2 x nb
-  double* KP;     // ceil(n/256)
-  double* part;   // ceil(n/256)
+  quat* Prow;     // matvec row partials: ceil(n/32) x n
+  quat* Pcol;     // matvec column partials: ceil(n/128) x n
+  quat* TS;       // tall-skinny partials: ceil(n/128) x 2 x nb
+  double* KP;     // ceil(n/64)
+  double* part;   // ceil(n/64)
   double* scal;   // 16
   quat* Tf;       // ceil((n-1)/nb) x nb x nb
+  quat* Gm;       // same shape: strict upper part of V^* V per panel
 };
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s);
 
