@@ -114,10 +114,10 @@ size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
   p->rw.W = b.take<quat>((size_t)n * nb);
   p->rw.ucur = b.take<quat>(nmax);
   p->rw.Prow = b.take<quat>((size_t)cdiv(nmax, 32) * n);
-  p->rw.Pcol = b.take<quat>((size_t)cdiv(nmax, 128) * n);
-  p->rw.TS = b.take<quat>((size_t)cdiv(nmax, 128) * 2 * nb);
-  p->rw.KP = b.take<double>(cdiv(nmax, 64) + 16);
-  p->rw.part = b.take<double>(cdiv(nmax, 64) + 16);
+  p->rw.Pcol = b.take<quat>((size_t)cdiv(nmax, 64) * n);
+  p->rw.TS = b.take<quat>((size_t)2 * nb);
+  p->rw.KP = b.take<double>(cdiv(nmax, 16) + 16);
+  p->rw.part = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.scal = b.take<double>(16);
   p->rw.Tf = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);
   p->rw.Gm = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);

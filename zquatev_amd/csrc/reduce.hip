@@ -23,10 +23,11 @@
 
 namespace zq {
 
-constexpr int RB = 256;      // threads per block
-constexpr int RR = 64;       // rows per block in the row-parallel kernels (4 waves split the K sums)
-constexpr int SW = 32;       // symv strip width (columns)
-constexpr int SG = 128;      // symv rows per block (2 per lane)
+constexpr int RB = 256;      // threads per block of the row kernels
+constexpr int RQ = 16;       // rows per block of the row kernels (16 threads per row split the K sums)
+constexpr int SW = 32;       // matvec strip width (columns)
+constexpr int SG = 64;       // matvec rows per block (one per lane)
+constexpr int SYT = 512;     // matvec threads: 8 waves x 4 columns
 constexpr int NBMAX = 128;   // largest supported panel width
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -41,43 +42,75 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Column c of the panel: apply the panel's pending update
+__device__ __forceinline__ void qatomic_add(quat* p, const quat& v) {
+  double* d = reinterpret_cast<double*>(p);
+  atomicAdd(d + 0, v.ar);
+  atomicAdd(d + 1, v.ai);
+  atomicAdd(d + 2, v.br);
+  atomicAdd(d + 3, v.bi);
+}
+
+// Column c of the panel (index j): apply the panel's pending update
 //   M[i,c] -= sum_{l<j} V(i,l) W(c,l)^* + W(i,l) V(c,l)^*     for i in [c, n)
 // and accumulate |M[i,c]|^2 over i >= c+2 (the reflector's tail norm).
 // Also finishes w_{j-1} = p_{j-1} - K u_{c-1} (K = u^*p / 2 from the previous
 // column's partials): each W(i, j-1), i > c, is read and rewritten by the one
-// thread that uses it; W(c, j-1) is formed on the fly (no later kernel reads it).
+// thread that uses it; W(c, j-1) is formed on the fly (no later kernel reads
+// it).  Zeroes the accumulator TS that the matvec of column c fills.
+// 16 threads per row, KPT panel columns each; every load is issued up front
+// at a clamped address.
+template <int KPT>
 __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                                    quat* __restrict__ W, int ldw, const double* __restrict__ KP,
-                                                   int nkp, double* __restrict__ part) {
+                                                   int nkp, double* __restrict__ part, quat* __restrict__ TS,
+                                                   int nb) {
   __shared__ quat cw[NBMAX], cv[NBMAX];
-  __shared__ quat red[4][RR];
+  __shared__ quat red[16][RQ];
   __shared__ double sh[RB / 64];
-  double K = 0.0;
-  if (j > 0) {
-    double t = 0.0;
-    for (int i = threadIdx.x; i < nkp; i += RB) t += KP[i];
-    K = 0.5 * block_sum(t, sh);
+  const int tid = threadIdx.x;
+  const int r = tid & (RQ - 1), q = tid >> 4;
+  const int i = c + blockIdx.x * RQ + r;
+  const int ic = min(i, n - 1);
+  const int jl = j > 0 ? j - 1 : 0;
+  const quat mcur = qload(M + ic + (size_t)c * ld);
+  const quat uprev = qload(M + ic + (size_t)(c > 0 ? c - 1 : 0) * ld);
+  quat vv[KPT], ww[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int l = min(q + 16 * k, jl);
+    vv[k] = qload(M + ic + (size_t)(p + l) * ld);
+    ww[k] = qload(W + ic + (size_t)l * ldw);
   }
-  for (int l = threadIdx.x; l < j; l += RB) {
-    quat wc = qload(W + c + (size_t)l * ldw);
-    if (l == j - 1) wc = qsub(wc, qscale(qload(M + c + (size_t)(c - 1) * ld), K));
-    cw[l] = qconj(wc);
-    cv[l] = qconj(qload(M + c + (size_t)(p + l) * ld));
+  double t = 0.0;
+  for (int k = tid; k < nkp; k += RB) t += KP[k];
+  quat cwl = qzero(), cvl = qzero(), ucc = qzero();
+  if (tid < j) {
+    cwl = qload(W + c + (size_t)tid * ldw);
+    cvl = qload(M + c + (size_t)(p + tid) * ld);
+    if (tid == j - 1) ucc = qload(M + c + (size_t)(c - 1) * ld);
+  }
+  // the matvec of this column accumulates into TS
+  if (blockIdx.x == 0) {
+    for (int k = tid; k < 2 * nb; k += RB) qstore(TS + k, qzero());
+  }
+  const double K = (j > 0) ? 0.5 * block_sum(t, sh) : 0.0;
+  if (tid < j) {
+    if (tid == j - 1) cwl = qsub(cwl, qscale(ucc, K));
+    cw[tid] = qconj(cwl);
+    cv[tid] = qconj(cvl);
   }
   __syncthreads();
-  const int r = threadIdx.x & (RR - 1), q = threadIdx.x >> 6;
-  const int i = c + blockIdx.x * RR + r;
   quat acc = qzero();
-  if (i < n) {
-#pragma unroll 4
-    for (int l = q; l < j; l += 4) {
-      quat wv = qload(W + i + (size_t)l * ldw);
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int l = q + 16 * k;
+    if (l < j) {
+      quat wv = ww[k];
       if (l == j - 1) {
-        wv = qsub(wv, qscale(qload(M + i + (size_t)(c - 1) * ld), K));
-        if (i > c) qstore(W + i + (size_t)l * ldw, wv);
+        wv = qsub(wv, qscale(uprev, K));
+        if (i > c && i < n) qstore(W + i + (size_t)l * ldw, wv);
       }
-      qfma(acc, qload(M + i + (size_t)(p + l) * ld), cw[l]);
+      qfma(acc, vv[k], cw[l]);
       qfma(acc, wv, cv[l]);
     }
   }
@@ -85,15 +118,18 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
   __syncthreads();
   double ss = 0.0;
   if (q == 0 && i < n) {
-    quat m = qload(M + i + (size_t)c * ld);
+    quat m = mcur;
     if (j > 0) {
-      m = qsub(m, qadd(qadd(red[0][r], red[1][r]), qadd(red[2][r], red[3][r])));
+      quat sum = qzero();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sum = qadd(sum, red[k][r]);
+      m = qsub(m, sum);
       qstore(M + i + (size_t)c * ld, m);
     }
     if (i >= c + 2) ss = qnorm2(m);
   }
   ss = block_sum(ss, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = ss;
+  if (tid == 0) part[blockIdx.x] = ss;
 }
 
 // Reflector for column c: x = M[c+1:, c].  H x = alpha e_1 with
@@ -104,8 +140,7 @@ struct Refl {
   double g;
   quat u1;
 };
-__device__ __forceinline__ Refl reflector(const quat* M, int ld, int c, double tail, quat* alpha) {
-  const quat x1 = qload(M + (c + 1) + (size_t)c * ld);
+__device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alpha) {
   const double ax1 = sqrt(qnorm2(x1));
   Refl R;
   if (!(tail > 0.0)) {
@@ -123,47 +158,25 @@ __device__ __forceinline__ Refl reflector(const quat* M, int ld, int c, double t
   return R;
 }
 
-__device__ __forceinline__ quat u_at(const quat* M, int ld, int c, int i, const Refl& R) {
-  const quat x = qscale(qload(M + i + (size_t)c * ld), R.g);
-  return (i == c + 1) ? R.u1 : x;
-}
-
-// Halving butterfly over a wave: on return lane l holds v[l >> (6 - log2 N)]...
-// specialised below for the two sizes used.
-template <int N>
-__device__ __forceinline__ double butterfly_sum(double* v) {
-  // N values per lane; after the exchange with masks 32, 16, ... down to 64/N
-  // each lane keeps one value, summed over the lanes that differ in those bits
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int m = 32, cnt = N / 2; cnt >= 1; m >>= 1, cnt >>= 1) {
-    const bool hi = (lane & m) != 0;
-#pragma unroll
-    for (int k = 0; k < cnt; ++k) {
-      const double send = hi ? v[k] : v[k + cnt];
-      const double keep = hi ? v[k + cnt] : v[k];
-      v[k] = keep + shfl_xor_d(send, m);
-    }
-  }
-  double r = v[0];
-  // remaining lane bits (below 64/N) still hold separate partial sums
-#pragma unroll
-  for (int m = 32 / N; m >= 1; m >>= 1) r += shfl_xor_d(r, m);
-  return r;
+// u_i from the raw column entry x_i = M[i, c]
+__device__ __forceinline__ quat u_from(const quat& x, int c, int i, const Refl& R) {
+  return (i == c + 1) ? R.u1 : qscale(x, R.g);
 }
 
 // Hermitian matvec y = M[s0:, s0:] u (s0 = c + 1) over the lower triangle.
 // Block (g, J) reads rows [s0 + SG g, +SG) x columns [s0 + SW J, +SW) (every
-// lower-triangle element exactly once over the grid): wave w of 8 takes 4 of
-// the 32 columns, each lane two rows, and all eight loads of a lane are in
-// flight together.  Row sums (over the 32 columns) go to Prow[J][row]; column
-// sums (over the 128 rows, reduced across the wave with a halving butterfly)
-// go to Pcol[g][col].  Then
-//   y(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i].
-// Blocks are enumerated row-group major: group g has 4 (g + 1) strips.
-// Second phase: the tall-skinny products W^* u and V^* u over the block's rows,
-// one group of 4 panel columns per block of the row group, into TS[g].
-constexpr int SYT = 512;     // threads of the matvec kernel
+// lower-triangle element exactly once over the grid); wave w of 8 takes 4 of
+// the 32 columns, one row per lane.  The matrix loads are issued first, so
+// the reflector set-up (tail-norm reduction, x_1) runs under their latency.
+// Row sums over the 32 columns (summed across the waves in LDS) go to
+// Prow[J][row]; column sums over the 64 rows (a VALU-only halving butterfly,
+// DPP and permlane swaps) go to Pcol[g][col]; then
+//   y(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
+// every (slot, row) written exactly once, so the sum is deterministic.
+// Blocks are enumerated row-group major: group g has 2 (g + 1) strips.
+// Second phase: the tall-skinny products W^* u and V^* u over the block's
+// rows, groups of 8 panel columns spread over the row group's blocks, added
+// into TS = [W^* u | V^* u] with FP64 atomics.
 __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                               const quat* __restrict__ W, int ldw, const double* __restrict__ part,
                                               int nparts, double* __restrict__ scal, double* __restrict__ d,
@@ -172,67 +185,59 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   __shared__ double sh[SYT / 64];
   __shared__ quat uJ[SW];
   __shared__ quat red[8][SG];
-  __shared__ quat tsr[8][4][2];
+  __shared__ quat tsr[8][8][2];
+  const int s0 = c + 1;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int b = blockIdx.x;
+  // block -> (g, J): g (g + 1) <= b < (g + 1) (g + 2)
+  int g = (int)((sqrt(4.0 * b + 1.0) - 1.0) * 0.5);
+  while ((g + 1) * (g + 2) <= b) ++g;
+  while (g * (g + 1) > b) --g;
+  const int J = b - g * (g + 1);
+  const int R0 = s0 + SG * g, C0 = s0 + SW * J;
+  const bool active = C0 < n;   // uniform for the block
+  const int ra = R0 + lane;
+  const int rac = min(ra, n - 1);
+  quat qa[4];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    const int cgc = min(C0 + 4 * w + cc, n - 1);
+    qa[cc] = qload(M + rac + (size_t)cgc * ld);
+  }
+  const quat xa = qload(M + rac + (size_t)c * ld);
+  const quat xj = qload(M + min(C0 + (tid & (SW - 1)), n - 1) + (size_t)c * ld);
+  const quat x1 = qload(M + (c + 1) + (size_t)c * ld);
   double tail = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += SYT) tail += part[i];
+  for (int k = tid; k < nparts; k += SYT) tail += part[k];
   tail = block_sum(tail, sh);
   quat alpha;
-  const Refl R = reflector(M, ld, c, tail, &alpha);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const Refl R = reflector(x1, tail, &alpha);
+  if (b == 0 && tid == 0) {
     scal[0] = R.g;
     scal[1] = R.u1.ar; scal[2] = R.u1.ai; scal[3] = R.u1.br; scal[4] = R.u1.bi;
     d[c] = M[c + (size_t)c * ld].ar;
     qstore(e + c, alpha);
   }
-  const int s0 = c + 1;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int b = blockIdx.x;
-  // block -> (g, J): 2 g (g + 1) <= b < 2 (g + 1) (g + 2)
-  int g = (int)((sqrt(2.0 * b + 1.0) - 1.0) * 0.5);
-  while (2 * (g + 1) * (g + 2) <= b) ++g;
-  while (2 * g * (g + 1) > b) --g;
-  const int J = b - 2 * g * (g + 1);
-  const int R0 = s0 + SG * g, C0 = s0 + SW * J;
-  const int tid = threadIdx.x;
-  const bool active = C0 < n;   // uniform for the block
-  if (active) {
-    if (tid < SW) uJ[tid] = (C0 + tid < n) ? u_at(M, ld, c, C0 + tid, R) : qzero();
-    if (J == 0 && tid < SG && R0 + tid < n) qstore(ucur + R0 + tid, u_at(M, ld, c, R0 + tid, R));
-  }
+  if (tid < SW) uJ[tid] = (C0 + tid < n) ? u_from(xj, c, C0 + tid, R) : qzero();
+  const quat ua = (ra < n) ? u_from(xa, c, ra, R) : qzero();
+  if (active && J == 0 && w == 0 && ra < n) qstore(ucur + ra, ua);
   __syncthreads();
   if (active) {
-    const int ra = R0 + lane, rb = R0 + 64 + lane;
-    const int rac = min(ra, n - 1), rbc = min(rb, n - 1);
-    quat qa[4], qb[4];
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-      const int cgc = min(C0 + 4 * w + cc, n - 1);
-      qa[cc] = qload(M + rac + (size_t)cgc * ld);
-      qb[cc] = qload(M + rbc + (size_t)cgc * ld);
-    }
-    const quat ua = (ra < n) ? u_at(M, ld, c, rac, R) : qzero();
-    const quat ub = (rb < n) ? u_at(M, ld, c, rbc, R) : qzero();
-    quat acca = qzero(), accb = qzero();
+    quat acc = qzero();
     double col[16];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
       const int cl = 4 * w + cc;
       const int cg = C0 + cl;
-      const bool okc = cg < n;
-      quat a = (okc && ra < n && ra >= cg) ? qa[cc] : qzero();
-      quat bq = (okc && rb < n && rb >= cg) ? qb[cc] : qzero();
+      quat a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
       if (ra == cg) a = quat{a.ar, 0.0, 0.0, 0.0};
-      if (rb == cg) bq = quat{bq.ar, 0.0, 0.0, 0.0};
-      const quat uc = uJ[cl];
-      qfma(acca, a, uc);
-      qfma(accb, bq, uc);
+      qfma(acc, a, uJ[cl]);
       quat cp = qzero();
       if (ra > cg) qfma_cl(cp, a, ua);
-      if (rb > cg) qfma_cl(cp, bq, ub);
       col[4 * cc + 0] = cp.ar; col[4 * cc + 1] = cp.ai; col[4 * cc + 2] = cp.br; col[4 * cc + 3] = cp.bi;
     }
-    red[w][lane] = acca;
-    red[w][64 + lane] = accb;
+    red[w][lane] = acc;
     // lane l ends with value (l >> 2) = component (l >> 2) & 3 of column (l >> 4), summed over the wave
     const double cs = butterfly_sum<16>(col);
     if ((lane & 3) == 0) {
@@ -244,102 +249,117 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   if (active && tid < SG) {
     const int i = R0 + tid;
     if (i < n && i >= C0) {
-      quat y = qzero();
+      quat sum = qzero();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) y = qadd(y, red[k][tid]);
-      qstore(Prow + (size_t)J * ldp + i, y);
+      for (int k = 0; k < 8; ++k) sum = qadd(sum, red[k][tid]);
+      qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
-  // tall-skinny phase: block (g, J) handles panel-column groups J, J + 4(g+1), ...
-  const int ngr = (j + 3) >> 2;
-  const int stride = 4 * (g + 1);
+  // tall-skinny phase: block (g, J) handles panel-column groups J, J + 2(g+1), ...
+  const int ngr = (j + 7) >> 3;
+  const int stride = 2 * (g + 1);
   for (int lg = J; lg < ngr; lg += stride) {
-    const int lo = tid & 3;
-    const int row = R0 + (tid >> 2);
-    const int l = 4 * lg + lo;
+    const int lo = tid & 7;
+    const int row = R0 + (tid >> 3);
+    const int l = 8 * lg + lo;
     quat aw = qzero(), av = qzero();
     if (row < n && l < j) {
-      const quat ur = u_at(M, ld, c, row, R);
+      const quat xr = qload(M + row + (size_t)c * ld);
+      const quat ur = u_from(xr, c, row, R);
       qfma_cl(aw, qload(W + row + (size_t)l * ldw), ur);
       qfma_cl(av, qload(M + row + (size_t)(p + l) * ld), ur);
     }
+    // sum over the 8 rows of the wave (lanes differing in bits 3..5)
+    double tv[8] = {aw.ar, aw.ai, aw.br, aw.bi, av.ar, av.ai, av.br, av.bi};
 #pragma unroll
-    for (int m = 4; m <= 32; m <<= 1) {
-      aw.ar += shfl_xor_d(aw.ar, m); aw.ai += shfl_xor_d(aw.ai, m);
-      aw.br += shfl_xor_d(aw.br, m); aw.bi += shfl_xor_d(aw.bi, m);
-      av.ar += shfl_xor_d(av.ar, m); av.ai += shfl_xor_d(av.ai, m);
-      av.br += shfl_xor_d(av.br, m); av.bi += shfl_xor_d(av.bi, m);
+    for (int k = 0; k < 8; ++k) {
+      double x = tv[k] + xlane<8>(tv[k]);
+      x = bf16(x, x);
+      tv[k] = bf32(x, x);
     }
-    if (lane < 4) {
+    aw = quat{tv[0], tv[1], tv[2], tv[3]};
+    av = quat{tv[4], tv[5], tv[6], tv[7]};
+    if (lane < 8) {
       tsr[w][lane][0] = aw;
       tsr[w][lane][1] = av;
     }
     __syncthreads();
-    if (tid < 32) {
-      const int side = tid >> 4, ll = (tid >> 2) & 3, comp = tid & 3;
+    if (tid < 64) {
+      const int side = tid >> 5, ll = (tid >> 2) & 7, comp = tid & 3;
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc += reinterpret_cast<const double*>(&tsr[k][ll][side])[comp];
-      if (4 * lg + ll < j) reinterpret_cast<double*>(TS + ((size_t)g * 2 + side) * nb + 4 * lg + ll)[comp] = acc;
+      if (8 * lg + ll < j) atomicAdd(reinterpret_cast<double*>(TS + (size_t)side * nb + 8 * lg + ll) + comp, acc);
     }
     __syncthreads();
   }
 }
 
-// p = sum of matvec partials - V (W^* u) - W (V^* u); stored in W[:, j].  Also
-// stores u into M[c+1:, c], partial u^* p into KP, and (block 0) the T-factor
-// column j: T[0:j, j] = -T[0:j, 0:j] (V^* u), T[j, j] = 1 (0 if u = 0) and
-// zeros below, so that H_p ... H_{p+nb-1} = I - V T V^* with T stored densely.
+// p = y - V (W^* u) - W (V^* u), y from the matvec partials; stored in W[:, j].  Also stores u into
+// M[c+1:, c], partial u^* p into KP, and (block 0) column j of the strict
+// upper part of G = V^* V (the T factor is formed from G after the panel).
+template <int KPT>
 __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                                 quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                                 const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
-                                                int nchunks, int nb, const quat* __restrict__ ucur,
+                                                int nb, const quat* __restrict__ ucur,
                                                 double* __restrict__ KP, quat* __restrict__ Gm) {
   __shared__ quat a[NBMAX], b[NBMAX];
-  __shared__ quat red[4][RR];
+  __shared__ quat red[16][RQ];
   __shared__ double sh[RB / 64];
-  for (int l = threadIdx.x; l < 2 * j; l += RB) {
-    const int side = l / j, ll = l - side * j;
-    quat s = qzero();
-#pragma unroll 8
-    for (int h = 0; h < nchunks; ++h) s = qadd(s, qload(TS + ((size_t)h * 2 + side) * nb + ll));
-    if (side == 0) a[ll] = s; else b[ll] = s;
+  const int tid = threadIdx.x;
+  const int s0 = c + 1;
+  const int r = tid & (RQ - 1), q = tid >> 4;
+  const int i = s0 + blockIdx.x * RQ + r;
+  const int ic = min(i, n - 1);
+  const int jl = j > 0 ? j - 1 : 0;
+  quat vv[KPT], ww[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int l = min(q + 16 * k, jl);
+    vv[k] = qload(M + ic + (size_t)(p + l) * ld);
+    ww[k] = qload(W + ic + (size_t)l * ldw);
+  }
+  const quat ui = qload(ucur + ic);
+  quat yi = qzero();
+  {
+    const int li = ic - s0;
+    const int G = (n - s0 + SG - 1) / SG;
+#pragma unroll 4
+    for (int J = q; J <= li / SW; J += 16) yi = qadd(yi, qload(Prow + (size_t)J * ldp + ic));
+#pragma unroll 4
+    for (int gg = li / SG + q; gg < G; gg += 16) yi = qadd(yi, qload(Pcol + (size_t)gg * ldp + ic));
+  }
+  if (tid < j) {
+    a[tid] = qload(TS + tid);
+    b[tid] = qload(TS + nb + tid);
   }
   __syncthreads();
   if (blockIdx.x == 0) {
-    // column j of G = V^* V (strict upper part): T = (I + triu(G, 1))^{-1} later
-    for (int r = threadIdx.x; r < nb; r += RB) qstore(Gm + r + (size_t)j * nb, r < j ? b[r] : qzero());
+    for (int rr = tid; rr < nb; rr += RB) qstore(Gm + rr + (size_t)j * nb, rr < j ? b[rr] : qzero());
   }
-  const int s0 = c + 1;
-  const int sz = n - s0;
-  const int G = (sz + SG - 1) / SG;
-  const int r = threadIdx.x & (RR - 1), q = threadIdx.x >> 6;
-  const int i = s0 + blockIdx.x * RR + r;
-  quat acc = qzero();
-  if (i < n) {
-    const int li = i - s0;
-#pragma unroll 4
-    for (int J = q; J <= li / SW; J += 4) acc = qadd(acc, qload(Prow + (size_t)J * ldp + i));
-#pragma unroll 4
-    for (int gg = li / SG + q; gg < G; gg += 4) acc = qadd(acc, qload(Pcol + (size_t)gg * ldp + i));
-#pragma unroll 4
-    for (int l = q; l < j; l += 4) {
-      acc = qsub(acc, qmul(qload(M + i + (size_t)(p + l) * ld), a[l]));
-      acc = qsub(acc, qmul(qload(W + i + (size_t)l * ldw), b[l]));
+  quat acc = yi;   // this thread's share of y(i) minus its share of the corrections
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int l = q + 16 * k;
+    if (l < j) {
+      acc = qsub(acc, qmul(vv[k], a[l]));
+      acc = qsub(acc, qmul(ww[k], b[l]));
     }
   }
   red[q][r] = acc;
   __syncthreads();
   double kp = 0.0;
   if (q == 0 && i < n) {
-    const quat y = qadd(qadd(red[0][r], red[1][r]), qadd(red[2][r], red[3][r]));
-    qstore(W + i + (size_t)j * ldw, y);
-    const quat u = qload(ucur + i);
-    qstore(M + i + (size_t)c * ld, u);
-    kp = u.ar * y.ar + u.ai * y.ai + u.br * y.br + u.bi * y.bi;
+    quat pv = qzero();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) pv = qadd(pv, red[k][r]);
+    qstore(W + i + (size_t)j * ldw, pv);
+    qstore(M + i + (size_t)c * ld, ui);
+    kp = ui.ar * pv.ar + ui.ai * pv.ai + ui.br * pv.br + ui.bi * pv.bi;
   }
   kp = block_sum(kp, sh);
-  if (threadIdx.x == 0) KP[blockIdx.x] = kp;
+  if (tid == 0) KP[blockIdx.x] = kp;
 }
 
 // w = p - (u^* p / 2) u  (last column of a panel; the others are finished by
@@ -392,6 +412,32 @@ __global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* _
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// panel columns per thread (16 threads per row) as a compile-time bound
+static int kpt_for(int nb) {
+  const int k = cdiv(nb, 16);
+  return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
+}
+
+template <class... Args>
+static void launch_col_update(int kpt, dim3 grid, hipStream_t s, Args... args) {
+  switch (kpt) {
+    case 1: hipLaunchKernelGGL(k_col_update<1>, grid, dim3(RB), 0, s, args...); break;
+    case 2: hipLaunchKernelGGL(k_col_update<2>, grid, dim3(RB), 0, s, args...); break;
+    case 4: hipLaunchKernelGGL(k_col_update<4>, grid, dim3(RB), 0, s, args...); break;
+    default: hipLaunchKernelGGL(k_col_update<8>, grid, dim3(RB), 0, s, args...); break;
+  }
+}
+
+template <class... Args>
+static void launch_wreduce(int kpt, dim3 grid, hipStream_t s, Args... args) {
+  switch (kpt) {
+    case 1: hipLaunchKernelGGL(k_wreduce<1>, grid, dim3(RB), 0, s, args...); break;
+    case 2: hipLaunchKernelGGL(k_wreduce<2>, grid, dim3(RB), 0, s, args...); break;
+    case 4: hipLaunchKernelGGL(k_wreduce<4>, grid, dim3(RB), 0, s, args...); break;
+    default: hipLaunchKernelGGL(k_wreduce<8>, grid, dim3(RB), 0, s, args...); break;
+  }
+}
+
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s) {
   if (n <= 0) return ZQI_OK;
   if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
@@ -400,6 +446,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
     return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
   }
   const int ldw = n;
+  const int kpt = kpt_for(nb);
   int panel = 0;
   for (int p = 0; p < n - 1; p += nb, ++panel) {
     const int wdt = (nb < n - 1 - p) ? nb : (n - 1 - p);
@@ -407,22 +454,20 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
     for (int j = 0; j < wdt; ++j) {
       const int c = p + j;
       const int nrow = n - c;                      // rows c..n-1
-      const int nblk = cdiv(nrow, RR);
-      const int nkp = cdiv(nrow, RR);              // blocks of the previous column's k_wreduce
-      hipLaunchKernelGGL(k_col_update, dim3(nblk), dim3(RB), 0, s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part);
+      const int nblk = cdiv(nrow, RQ);
+      const int nkp = cdiv(nrow, RQ);              // blocks of the previous column's k_wreduce
+      launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part, w.TS, nb);
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int nmain = 2 * G * (G + 1);
-      const int nch = (j > 0) ? G : 0;             // tall-skinny partials: one per row group
+      const int nmain = G * (G + 1);
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
         hipLaunchKernelGGL(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
                            w.scal, d, e, w.Prow, w.Pcol, n, w.ucur, w.TS, nb);
       }
-      const int nb2 = cdiv(sz, RR);
-      hipLaunchKernelGGL(k_wreduce, dim3(nb2), dim3(RB), 0, s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS,
-                         nch, nb, w.ucur, w.KP, Gp);
+      const int nb2 = cdiv(sz, RQ);
+      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.ucur, w.KP, Gp);
       if (j == wdt - 1)
         hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2);
     }

@@ -77,20 +77,101 @@ __device__ __forceinline__ void qfma_cr(quat& acc, const quat& x, const quat& y)
 __device__ __forceinline__ double shfl_xor_d(double v, int m) {
   return __shfl_xor(v, m, 64);
 }
-__device__ __forceinline__ quat wave_sum(quat v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    v.ar += shfl_xor_d(v.ar, m);
-    v.ai += shfl_xor_d(v.ai, m);
-    v.br += shfl_xor_d(v.br, m);
-    v.bi += shfl_xor_d(v.bi, m);
-  }
-  return v;
+
+// ---- cross-lane exchanges without the LDS crossbar (gfx950) ----------------
+// xor 1, 2, 4, 8 inside a 16-lane row through DPP; the row-crossing steps of a
+// butterfly through v_permlane16_swap / v_permlane32_swap.
+__device__ __forceinline__ unsigned lo32(double v) { return (unsigned)(unsigned long long)__double_as_longlong(v); }
+__device__ __forceinline__ unsigned hi32(double v) {
+  return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32);
 }
-__device__ __forceinline__ double wave_sum_d(double v) {
+__device__ __forceinline__ double mkd(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppd(double v) {
+  return mkd(__builtin_amdgcn_update_dpp(0u, lo32(v), CTRL, 0xF, 0xF, false),
+             __builtin_amdgcn_update_dpp(0u, hi32(v), CTRL, 0xF, 0xF, false));
+}
+// xor 4: row_shl:4 on banks 0 and 2, row_shr:4 on banks 1 and 3
+__device__ __forceinline__ double dpp_xor4(double v) {
+  unsigned l = __builtin_amdgcn_update_dpp(0u, lo32(v), 0x104, 0xF, 0x5, false);
+  l = __builtin_amdgcn_update_dpp(l, lo32(v), 0x114, 0xF, 0xA, false);
+  unsigned h = __builtin_amdgcn_update_dpp(0u, hi32(v), 0x104, 0xF, 0x5, false);
+  h = __builtin_amdgcn_update_dpp(h, hi32(v), 0x114, 0xF, 0xA, false);
+  return mkd(l, h);
+}
+// value of v in lane (lane ^ M), M in {1, 2, 4, 8}
+template <int M>
+__device__ __forceinline__ double xlane(double v) {
+  if (M == 8) return dppd<0x128>(v);   // row_ror:8
+  if (M == 4) return dpp_xor4(v);
+  if (M == 2) return dppd<0x4E>(v);    // quad_perm [2,3,0,1]
+  return dppd<0xB1>(v);                // quad_perm [1,0,3,2]
+}
+// Butterfly exchange across 32 lanes: lanes 0..31 return A + A(lane^32),
+// lanes 32..63 return B(lane^32) + B.
+__device__ __forceinline__ double bf32(double A, double B) {
+  auto l = __builtin_amdgcn_permlane32_swap(lo32(A), lo32(B), false, false);
+  auto h = __builtin_amdgcn_permlane32_swap(hi32(A), hi32(B), false, false);
+  return mkd(l[0], h[0]) + mkd(l[1], h[1]);
+}
+// Same across 16 lanes (even rows keep A, odd rows keep B).
+__device__ __forceinline__ double bf16(double A, double B) {
+  auto l = __builtin_amdgcn_permlane16_swap(lo32(A), lo32(B), false, false);
+  auto h = __builtin_amdgcn_permlane16_swap(hi32(A), hi32(B), false, false);
+  return mkd(l[0], h[0]) + mkd(l[1], h[1]);
+}
+template <int M, int CNT>
+__device__ __forceinline__ void bstep(double* v, int lane) {
+  if (M == 32) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
-  return v;
+    for (int k = 0; k < CNT; ++k) v[k] = bf32(v[k], v[k + CNT]);
+  } else if (M == 16) {
+#pragma unroll
+    for (int k = 0; k < CNT; ++k) v[k] = bf16(v[k], v[k + CNT]);
+  } else {
+    const bool hi = (lane & M) != 0;
+#pragma unroll
+    for (int k = 0; k < CNT; ++k) {
+      const double send = hi ? v[k] : v[k + CNT];
+      const double keep = hi ? v[k + CNT] : v[k];
+      v[k] = keep + xlane<M>(send);
+    }
+  }
+}
+// Halving butterfly over a wave for N (power of two <= 64) values per lane:
+// lane l returns value (l >> (6 - log2 N)) summed over all 64 lanes.
+template <int N>
+__device__ __forceinline__ double butterfly_sum(double* v) {
+  const int lane = threadIdx.x & 63;
+  if (N >= 2) bstep<32, (N >= 2 ? N / 2 : 1)>(v, lane);
+  if (N >= 4) bstep<16, (N >= 4 ? N / 4 : 1)>(v, lane);
+  if (N >= 8) bstep<8, (N >= 8 ? N / 8 : 1)>(v, lane);
+  if (N >= 16) bstep<4, (N >= 16 ? N / 16 : 1)>(v, lane);
+  if (N >= 32) bstep<2, (N >= 32 ? N / 32 : 1)>(v, lane);
+  if (N >= 64) bstep<1, 1>(v, lane);
+  // lane bits not consumed by the halving steps still hold separate partial sums
+  double r = v[0];
+  if (N <= 2) r = bf16(r, r);
+  if (N <= 4) r += xlane<8>(r);
+  if (N <= 8) r += xlane<4>(r);
+  if (N <= 16) r += xlane<2>(r);
+  if (N <= 32) r += xlane<1>(r);
+  if (N == 1) r = bf32(r, r);
+  return r;
+}
+// sum over all 64 lanes, result in every lane
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += xlane<1>(v);
+  v += xlane<2>(v);
+  v += xlane<4>(v);
+  v += xlane<8>(v);
+  v = bf16(v, v);
+  return bf32(v, v);
+}
+__device__ __forceinline__ quat wave_sum(quat v) {
+  return quat{wave_sum_d(v.ar), wave_sum_d(v.ai), wave_sum_d(v.br), wave_sum_d(v.bi)};
 }
 
 }  // namespace zq
@@ -132,10 +213,10 @@ struct ReduceWork {
   quat* W;        // n x nb (global row index)
   quat* ucur;     // n
   quat* Prow;     // matvec row partials: ceil(n/32) x n
-  quat* Pcol;     // matvec column partials: ceil(n/128) x n
-  quat* TS;       // tall-skinny partials: ceil(n/128) x 2 x nb
-  double* KP;     // ceil(n/64)
-  double* part;   // ceil(n/64)
+  quat* Pcol;     // matvec column partials: ceil(n/64) x n
+  quat* TS;       // tall-skinny accumulators [W^* u | V^* u]: 2 x nb
+  double* KP;     // ceil(n/16)
+  double* part;   // ceil(n/16)
   double* scal;   // 16
   quat* Tf;       // ceil((n-1)/nb) x nb x nb
   quat* Gm;       // same shape: strict upper part of V^* V per panel
