@@ -20,6 +20,7 @@
 // bytes for an m x m trailing block).
 #include "zq_internal.h"
 #include <climits>
+#include <algorithm>
 
 namespace zq {
 
@@ -40,14 +41,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int i = 0; i < nw; ++i) r += sh[i];
   __syncthreads();
   return r;
-}
-
-__device__ __forceinline__ void qatomic_add(quat* p, const quat& v) {
-  double* d = reinterpret_cast<double*>(p);
-  atomicAdd(d + 0, v.ar);
-  atomicAdd(d + 1, v.ai);
-  atomicAdd(d + 2, v.br);
-  atomicAdd(d + 3, v.bi);
 }
 
 // Column c of the panel (index j): apply the panel's pending update
@@ -196,7 +189,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   while (g * (g + 1) > b) --g;
   const int J = b - g * (g + 1);
   const int R0 = s0 + SG * g, C0 = s0 + SW * J;
-  const bool active = C0 < n;   // uniform for the block
+  const bool active = C0 < n && R0 < n;   // uniform for the block
   const int ra = R0 + lane;
   const int rac = min(ra, n - 1);
   quat qa[4];
@@ -208,6 +201,14 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   const quat xa = qload(M + rac + (size_t)c * ld);
   const quat xj = qload(M + min(C0 + (tid & (SW - 1)), n - 1) + (size_t)c * ld);
   const quat x1 = qload(M + (c + 1) + (size_t)c * ld);
+  // tall-skinny task of this block (at most one): rows of group tg, panel columns 8 tl .. 8 tl + 7
+  const int ngr = (j + 7) >> 3;
+  const int Gt = (n - s0 + SG - 1) / SG;
+  const bool has_ts = b < Gt * ngr;
+  const int tg = has_ts ? b / ngr : 0, tl = has_ts ? b - (b / ngr) * ngr : 0;
+  const int trow = s0 + SG * tg + (tid >> 3);
+  const int tcol = 8 * tl + (tid & 7);
+  const int trc = min(trow, n - 1), tcc = min(tcol, j > 0 ? j - 1 : 0);
   double tail = 0.0;
   for (int k = tid; k < nparts; k += SYT) tail += part[k];
   tail = block_sum(tail, sh);
@@ -255,33 +256,29 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
-  // tall-skinny phase: block (g, J) handles panel-column groups J, J + 2(g+1), ...
-  const int ngr = (j + 7) >> 3;
-  const int stride = 2 * (g + 1);
-  for (int lg = J; lg < ngr; lg += stride) {
-    const int lo = tid & 7;
-    const int row = R0 + (tid >> 3);
-    const int l = 8 * lg + lo;
+  // tall-skinny task: W^* u and V^* u over 64 rows for 8 panel columns
+  if (has_ts) {
+    const quat tw = qload(W + trc + (size_t)tcc * ldw);
+    const quat tv = qload(M + trc + (size_t)(p + tcc) * ld);
+    const quat tx = qload(M + trc + (size_t)c * ld);
+    const bool ok = trow < n && tcol < j;
     quat aw = qzero(), av = qzero();
-    if (row < n && l < j) {
-      const quat xr = qload(M + row + (size_t)c * ld);
-      const quat ur = u_from(xr, c, row, R);
-      qfma_cl(aw, qload(W + row + (size_t)l * ldw), ur);
-      qfma_cl(av, qload(M + row + (size_t)(p + l) * ld), ur);
+    if (ok) {
+      const quat ur = u_from(tx, c, trow, R);
+      qfma_cl(aw, tw, ur);
+      qfma_cl(av, tv, ur);
     }
     // sum over the 8 rows of the wave (lanes differing in bits 3..5)
-    double tv[8] = {aw.ar, aw.ai, aw.br, aw.bi, av.ar, av.ai, av.br, av.bi};
+    double t8[8] = {aw.ar, aw.ai, aw.br, aw.bi, av.ar, av.ai, av.br, av.bi};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      double x = tv[k] + xlane<8>(tv[k]);
+      double x = t8[k] + xlane<8>(t8[k]);
       x = bf16(x, x);
-      tv[k] = bf32(x, x);
+      t8[k] = bf32(x, x);
     }
-    aw = quat{tv[0], tv[1], tv[2], tv[3]};
-    av = quat{tv[4], tv[5], tv[6], tv[7]};
     if (lane < 8) {
-      tsr[w][lane][0] = aw;
-      tsr[w][lane][1] = av;
+      tsr[w][lane][0] = quat{t8[0], t8[1], t8[2], t8[3]};
+      tsr[w][lane][1] = quat{t8[4], t8[5], t8[6], t8[7]};
     }
     __syncthreads();
     if (tid < 64) {
@@ -289,9 +286,8 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc += reinterpret_cast<const double*>(&tsr[k][ll][side])[comp];
-      if (8 * lg + ll < j) atomicAdd(reinterpret_cast<double*>(TS + (size_t)side * nb + 8 * lg + ll) + comp, acc);
+      if (8 * tl + ll < j) atomicAdd(reinterpret_cast<double*>(TS + (size_t)side * nb + 8 * tl + ll) + comp, acc);
     }
-    __syncthreads();
   }
 }
 
@@ -459,7 +455,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part, w.TS, nb);
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int nmain = G * (G + 1);
+      const int nmain = std::max(G * (G + 1), G * ((j + 7) / 8));
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
