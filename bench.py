@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
                     help="only warmup + timed steps (for rocprofv3), no instrumentation pass or baseline")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="with --profile-run: also run one elementwise copy (torch.neg) of MIB MiB, "
+                         "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-child", nargs=2, type=int, metavar=("N2", "THREADS"), help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -88,6 +91,21 @@ def cpu_baseline(n2):
             "cores": r["threads"], "kind": r["kind"],
             "sample": (f"one ts::zquatev call (eigenvalues+eigenvectors) at 2n={n2}, same U(-1,1) generator, "
                        f"{r['threads']} MKL threads, {r['seconds']:.2f} s; GFLOP/s on (160/3) n^3")}
+
+
+def measured_traffic(kernel, n2, nb):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
+    scaled by a known-size copy), or None when no summary covers it."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_2n{n2}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if f"panel width {nb}" not in text:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -144,7 +162,16 @@ def main():
     value = flops / (ms_per_step / 1000.0) / 1e9
 
     if args.profile_run:
-        print(json.dumps({"ms_per_step": ms_per_step, "gflops": value, "phases_ms": phases}))
+        out = {"ms_per_step": ms_per_step, "gflops": value, "phases_ms": phases}
+        if args.calib_copy > 0:
+            nel = args.calib_copy * (1 << 20) // 8
+            src = torch.ones(nel, dtype=torch.float64, device=dev)
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            torch.neg(src, out=dst)
+            torch.cuda.synchronize()
+            out["calib_copy_bytes_each_way"] = nel * 8
+        print(json.dumps(out))
         return 0
 
     check = None
@@ -185,7 +212,10 @@ def main():
     if dom == "symv":
         ach = kd["units"] / (kd["total_ms"] / 1000.0) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_symv (panel Hermitian matvec)"}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_symv", n2, nb),
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                "algorithmic_bytes_per_launch": kd["units"] / kd["launches"],
+                "kernel": "k_symv (panel Hermitian matvec)"}
     else:
         ach = kd["units"] / (kd["total_ms"] / 1000.0) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",

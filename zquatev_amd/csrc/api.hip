@@ -49,6 +49,8 @@ struct zq_context {
 
 
 namespace zq {
+static thread_local int g_pending = -1;   // slot armed by the innermost ProfScope
+
 ProfScope::ProfScope(int c, hipStream_t st, double u) : cls(c), s(st), units(u), slot(-1) {
   ProfState* p = g_prof;
   if (!p || !p->on) return;
@@ -63,11 +65,22 @@ ProfScope::ProfScope(int c, hipStream_t st, double u) : cls(c), s(st), units(u),
   if (p->cls.size() < p->used) { p->cls.resize(p->used); p->units.resize(p->used); }
   p->cls[slot] = c;
   p->units[slot] = u;
-  (void)hipEventRecord(p->pool[2 * slot], s);
+  g_pending = slot;
 }
 ProfScope::~ProfScope() {
   if (slot < 0) return;
-  (void)hipEventRecord(g_prof->pool[2 * slot + 1], s);
+  if (g_pending == slot) {
+    // no kernel was launched in the scope: an empty interval
+    g_pending = -1;
+    (void)hipEventRecord(g_prof->pool[2 * slot], s);
+    (void)hipEventRecord(g_prof->pool[2 * slot + 1], s);
+  }
+}
+void prof_take(hipEvent_t* e0, hipEvent_t* e1) {
+  if (g_pending < 0 || !g_prof) return;
+  *e0 = g_prof->pool[2 * g_pending];
+  *e1 = g_prof->pool[2 * g_pending + 1];
+  g_pending = -1;
 }
 }  // namespace zq
 

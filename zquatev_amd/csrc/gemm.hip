@@ -151,7 +151,7 @@ int qgemm(int m, int nn, int k, double alpha, const QOperand& A, const QOperand&
           bool lower_only, hipStream_t s) {
   if (m <= 0 || nn <= 0) return ZQI_OK;
   dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, 1);
-  hipLaunchKernelGGL(k_qgemm, grid, dim3(256), 0, s, m, nn, k, alpha, A, B, beta, C, ldc, lower_only ? 1 : 0, k, 0);
+  launch_k(k_qgemm, grid, dim3(256), 0, s, m, nn, k, alpha, A, B, beta, C, ldc, lower_only ? 1 : 0, k, 0);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -172,7 +172,7 @@ int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int 
   const int nz = splitk_count(K, nsplit, &kchunk);
   QOperand Bo{B, B, ldb, ldb, INT_MAX, false, INT_MIN / 2};
   dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, nz);
-  hipLaunchKernelGGL(k_qgemm, grid, dim3(256), 0, s, m, nn, K, 1.0, A, Bo, 0.0, Ypart, m, 0, kchunk,
+  launch_k(k_qgemm, grid, dim3(256), 0, s, m, nn, K, 1.0, A, Bo, 0.0, Ypart, m, 0, kchunk,
                      (size_t)m * nn);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
@@ -244,7 +244,7 @@ int dgemm_nn(int m, int, const int* k_dev, int k_max, const double* A, int lda, 
              int ldc, const int* cmap, hipStream_t s) {
   if (m <= 0 || k_max <= 0) return ZQI_OK;
   dim3 grid((m + DBM - 1) / DBM, (k_max + DBN - 1) / DBN, 1);
-  hipLaunchKernelGGL(k_dgemm_nn, grid, dim3(256), 0, s, m, k_dev, A, lda, B, ldb, C, ldc, cmap);
+  launch_k(k_dgemm_nn, grid, dim3(256), 0, s, m, k_dev, A, lda, B, ldb, C, ldc, cmap);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

@@ -459,7 +459,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
-        hipLaunchKernelGGL(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
+        launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
                            w.scal, d, e, w.Prow, w.Pcol, n, w.ucur, w.TS, nb);
       }
       const int nb2 = cdiv(sz, RQ);

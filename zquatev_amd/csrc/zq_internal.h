@@ -11,6 +11,7 @@
 //     conjugate (a,b)^* = (conj a, -b); |q|^2 = |a|^2 + |b|^2.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <cstddef>
 
@@ -195,7 +196,10 @@ namespace zq {
 
 // Optional per-kernel-class timing with HIP events (enabled per context).
 // Classes: symv (panel matvec), trail (trailing MFMA update), backtr (MFMA
-// back-transform products), merge (D&C merge GEMM), other.
+// back-transform products), merge (D&C merge GEMM), other.  A ProfScope arms
+// one event pair; the first launch_k inside the scope hands the pair to
+// hipExtLaunchKernelGGL, so the recorded interval is exactly that kernel's
+// execution (no launch gaps).
 enum ProfClass { PROF_SYMV = 0, PROF_TRAIL = 1, PROF_BACKTR = 2, PROF_MERGE = 3, PROF_NCLASS = 4 };
 struct ProfScope {
   int cls;
@@ -205,6 +209,15 @@ struct ProfScope {
   ProfScope(int cls, hipStream_t s, double units);
   ~ProfScope();
 };
+// events of the armed scope (nullptr when none); disarms it
+void prof_take(hipEvent_t* e0, hipEvent_t* e1);
+
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  prof_take(&e0, &e1);
+  hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, e0, e1, 0u, args...);
+}
 
 // reduce.hip: quaternion Householder tridiagonalisation of the lower triangle
 // of M (n x n, ld).  Writes d (n), e (n-1 quaternions: the subdiagonal),
