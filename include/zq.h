@@ -74,6 +74,28 @@ int zq_zquatev_ctx(zq_context* ctx, int n2, double* D, int nld2, double* eig);
  * context's stream. */
 int zq_zquatev_device(zq_context* ctx, int n2, double* dD, int nld2, double* deig, int jobz);
 
+/* Batched solves of independent problems of uneven sizes in one call.  The
+ * reference solves one matrix per call (zquatev.cc:42-72); a caller with many
+ * matrices loops over ts::zquatev.  Problem i is (n2[i], D[i], ld2[i],
+ * eig[i]) with the meaning of zq_zquatev_device's arguments; n2[i] = 0 is
+ * skipped.  Problems are grouped by panel width and each group runs its
+ * columns in lockstep, so every problem's result is bitwise the result of its
+ * own zq_zquatev_device call on the same context.
+ * Device-memory variant: asynchronous on the context's stream unless info is
+ * given (host array of count ints), in which case it waits and stores each
+ * problem's status there (0, or > 0: tridiagonal solver did not converge).
+ * Returns 0; ZQ_ERR_ARG for a null context; -2 count < 0; -3..-6 a null n2,
+ * D, ld2 or eig array; -7 bad jobz; -8 some problem's own arguments are
+ * invalid (nothing is run; info[i] holds that problem's zq_zquatev code,
+ * -1..-4); 1 if some problem did not converge; ZQ_ERR_HIP / ZQ_ERR_NOMEM. */
+int zq_zquatev_batched_device(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,
+                              double* const* eig, int jobz, int* info);
+
+/* Host-memory batch with eigenvectors (ts::zquatev semantics per problem),
+ * synchronous.  Same return codes; info optional. */
+int zq_zquatev_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,
+                       double* const* eig, int* info);
+
 /* Component entry points (device memory, asynchronous).
  *
  * zq_tridiagonalize: the symplectic reduction of the reference's panel loop

@@ -61,6 +61,14 @@ def test_argument_validation_without_device():
     assert info == -1
 
 
+def test_batched_entry_points_validate_without_device():
+    i32 = ctypes.c_int * 1
+    f = zq.lib().zq_zquatev_batched_device
+    assert f(None, 1, i32(4), None, i32(4), None, ord("V"), None) == zq.ERR_ARG   # null context
+    g = zq.lib().zq_zquatev_batched
+    assert g(None, 0, None, None, None, None, None) == zq.ERR_ARG
+
+
 def test_error_strings_and_tuning_table():
     assert zq.error_string(0) == "success"
     assert "odd" in zq.error_string(-1)

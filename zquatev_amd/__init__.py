@@ -33,6 +33,8 @@ _SIGNATURES = {
     "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
     "zq_zquatev_ctx": (_c_int, [_vp, _c_int, _vp, _c_int, _vp]),
     "zq_zquatev_device": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int]),
+    "zq_zquatev_batched_device": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    "zq_zquatev_batched": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "zq_tridiagonalize": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
     "zq_tridiag_eig": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
     "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
@@ -192,6 +194,32 @@ class Context:
     # device-pointer entry points: pointers are integers (e.g. torch .data_ptr())
     def zquatev_device(self, n2, d_ptr, ld2, eig_ptr, jobz="V"):
         return _fn("zq_zquatev_device")(self._h, int(n2), _vp(d_ptr), int(ld2), _vp(eig_ptr), ord(jobz))
+
+    def zquatev_batched(self, Ds):
+        """Host-memory batch: Ds is a list of column-major complex128 arrays
+        (each solved in place, as :meth:`zquatev`).  Returns (rc, infos, eigs)."""
+        Ds = [_fortran_c128(D) for D in Ds]
+        cnt = len(Ds)
+        n2 = (_c_int * max(cnt, 1))(*[D.shape[1] for D in Ds])
+        ld2 = (_c_int * max(cnt, 1))(*[D.shape[0] for D in Ds])
+        eigs = [np.zeros(max(D.shape[1] // 2, 1), dtype=np.float64) for D in Ds]
+        dp = (_vp * max(cnt, 1))(*[D.ctypes.data for D in Ds])
+        ep = (_vp * max(cnt, 1))(*[e.ctypes.data for e in eigs])
+        info = (_c_int * max(cnt, 1))()
+        rc = _fn("zq_zquatev_batched")(self._h, cnt, n2, dp, ld2, ep, info)
+        return rc, list(info)[:cnt], [e[: D.shape[1] // 2] for e, D in zip(eigs, Ds)]
+
+    def zquatev_batched_device(self, n2s, d_ptrs, ld2s, eig_ptrs, jobz="V", want_info=False):
+        """Device-pointer batch (integers, e.g. torch .data_ptr()).  Returns rc,
+        or (rc, infos) with want_info (which waits for completion)."""
+        cnt = len(n2s)
+        n2 = (_c_int * max(cnt, 1))(*[int(v) for v in n2s])
+        ld2 = (_c_int * max(cnt, 1))(*[int(v) for v in ld2s])
+        dp = (_vp * max(cnt, 1))(*[int(v) for v in d_ptrs])
+        ep = (_vp * max(cnt, 1))(*[int(v) for v in eig_ptrs])
+        info = (_c_int * max(cnt, 1))() if want_info else None
+        rc = _fn("zq_zquatev_batched_device")(self._h, cnt, n2, dp, ld2, ep, ord(jobz), info)
+        return (rc, list(info)[:cnt]) if want_info else rc
 
     def tridiagonalize_device(self, n2, d_ptr, ld2, dd_ptr, de_ptr):
         return _fn("zq_tridiagonalize")(self._h, int(n2), _vp(d_ptr), int(ld2), _vp(dd_ptr), _vp(de_ptr))

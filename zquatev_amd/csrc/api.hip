@@ -12,6 +12,7 @@
 #include "zq.h"
 #include "zquatev.h"
 
+#include <algorithm>
 #include <complex>
 #include <cstring>
 #include <mutex>
@@ -45,6 +46,7 @@ struct zq_context {
   bool have_times = false;
   int* info_dev = nullptr;  // leaf-solver status of the last vectors solve
   zq::ProfState prof;
+  zq::Staging stage;
 };
 
 
@@ -82,6 +84,57 @@ void prof_take(hipEvent_t* e0, hipEvent_t* e1) {
   *e1 = g_prof->pool[2 * g_pending + 1];
   g_pending = -1;
 }
+int stage_begin(Staging& st) {
+  st.slot ^= 1;
+  st.used = 0;
+  if (st.done[st.slot] && hipEventSynchronize(st.done[st.slot]) != hipSuccess) return ZQI_ERR_HIP;
+  return ZQI_OK;
+}
+
+int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStream_t s) {
+  if (bytes == 0) return ZQI_OK;
+  const int k = st.slot;
+  size_t off = (st.used + 255) & ~size_t(255);
+  if (off + bytes > st.cap[k]) {
+    // grow: every copy already issued from this buffer must have completed
+    if (st.host[k]) {
+      if (hipStreamSynchronize(s) != hipSuccess) return ZQI_ERR_HIP;
+      (void)hipHostFree(st.host[k]);
+      st.host[k] = nullptr;
+      st.cap[k] = 0;
+    }
+    const size_t cap = std::max<size_t>(size_t(1) << 20, 2 * (off + bytes));
+    void* h = nullptr;
+    if (hipHostMalloc(&h, cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return ZQI_ERR_NOMEM;
+    }
+    st.host[k] = static_cast<char*>(h);
+    st.cap[k] = cap;
+    off = 0;
+  }
+  std::memcpy(st.host[k] + off, src, bytes);
+  if (hipMemcpyAsync(dst, st.host[k] + off, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZQI_ERR_HIP;
+  st.used = off + bytes;
+  return ZQI_OK;
+}
+
+int stage_end(Staging& st, hipStream_t s) {
+  const int k = st.slot;
+  if (!st.done[k] && hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming) != hipSuccess) return ZQI_ERR_HIP;
+  return hipEventRecord(st.done[k], s) == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+void stage_free(Staging& st) {
+  for (int k = 0; k < 2; ++k) {
+    if (st.done[k]) (void)hipEventSynchronize(st.done[k]);
+    if (st.host[k]) (void)hipHostFree(st.host[k]);
+    if (st.done[k]) (void)hipEventDestroy(st.done[k]);
+    st.host[k] = nullptr;
+    st.done[k] = nullptr;
+    st.cap[k] = 0;
+  }
+}
 }  // namespace zq
 
 namespace {
@@ -115,9 +168,9 @@ struct Bump {
   }
 };
 
-// Lays out (or sizes, when base is null) the workspace of one solve.
-size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
-  Bump b{base};
+// The per-problem buffers of the reduction and back-transform (a batch
+// repeats this block once per problem, at a fixed stride).
+void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   const size_t nn = (size_t)n * n;
   const int nmax = n > 1 ? n : 2;
   p->M = b.take<quat>(nn);
@@ -129,40 +182,14 @@ size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
   p->rw.Prow = b.take<quat>((size_t)cdiv(nmax, 32) * n);
   p->rw.Pcol = b.take<quat>((size_t)cdiv(nmax, 64) * n);
   p->rw.TS = b.take<quat>((size_t)2 * nb);
+  p->rw.ldtsp = 8 * nb;
+  p->rw.TSP = b.take<double>((size_t)(cdiv(nmax, 16) + 1) * 8 * nb);
   p->rw.KP = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.part = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.scal = b.take<double>(16);
   p->rw.Tf = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);
   p->rw.Gm = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);
-  zq::StedcWork& s = p->sw;
-  s.lam = b.take<double>(nmax);
-  s.es = b.take<double>(nmax);
-  s.e2 = b.take<double>(nmax);
-  s.scal = b.take<double>(16);
   if (vectors) {
-    s.Z = b.take<double>(nn);
-    s.Qtmp = b.take<double>(nn);
-    s.U = b.take<double>(nn);
-    s.dk = b.take<double>(nmax);
-    s.zk = b.take<double>(nmax);
-    s.dkk = b.take<double>(nmax);
-    s.zkk = b.take<double>(nmax);
-    s.dval = b.take<double>(nmax);
-    s.tau = b.take<double>(nmax);
-    s.zhat = b.take<double>(nmax);
-    s.rho = b.take<double>(nmax);
-    s.rot = b.take<double>((size_t)4 * nmax);
-    s.src = b.take<int>(nmax);
-    s.keep = b.take<int>(nmax);
-    s.dfl = b.take<int>(nmax);
-    s.org = b.take<int>(nmax);
-    s.posK = b.take<int>(nmax);
-    s.posD = b.take<int>(nmax);
-    s.nrot = b.take<int>(nmax);
-    s.kcnt = b.take<int>(nmax);
-    s.info = b.take<int>(16);
-    s.int_slots = zq::stedc_int_slots(n);
-    s.ints = b.take<int>(s.int_slots);
     p->X = b.take<quat>(nn);
     p->bw.phi = b.take<quat>(nmax);
     p->bw.nsplit = kNSplit;
@@ -171,9 +198,55 @@ size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
     p->bw.Y = b.take<quat>((size_t)nb * n);
     p->bw.Y2 = b.take<quat>((size_t)nb * n);
   } else {
-    s.Z = s.Qtmp = s.U = nullptr;
     p->X = nullptr;
   }
+}
+
+// Divide & conquer buffers for a forest of nprob tridiagonals of total size
+// ntot, the largest lq.
+void stedc_layout(Bump& b, int ntot, int nprob, int lq, bool vectors, zq::StedcWork* sw) {
+  zq::StedcWork& s = *sw;
+  const int nt = ntot > 1 ? ntot : 2;
+  s.lam = b.take<double>(nt);
+  s.es = b.take<double>(nt);
+  s.e2 = b.take<double>(nt);
+  s.scal = b.take<double>(std::max(16, nprob));
+  s.int_slots = zq::stedc_int_slots(ntot, nprob);
+  s.ints = b.take<int>(s.int_slots);
+  s.info = b.take<int>(std::max(16, nprob));
+  s.stage = nullptr;
+  if (vectors) {
+    const size_t zz = (size_t)nt * lq;
+    s.Z = b.take<double>(zz);
+    s.Qtmp = b.take<double>(zz);
+    s.U = b.take<double>(zz);
+    s.dk = b.take<double>(nt);
+    s.zk = b.take<double>(nt);
+    s.dkk = b.take<double>(nt);
+    s.zkk = b.take<double>(nt);
+    s.dval = b.take<double>(nt);
+    s.tau = b.take<double>(nt);
+    s.zhat = b.take<double>(nt);
+    s.rho = b.take<double>(nt);
+    s.rot = b.take<double>((size_t)4 * nt);
+    s.src = b.take<int>(nt);
+    s.keep = b.take<int>(nt);
+    s.dfl = b.take<int>(nt);
+    s.org = b.take<int>(nt);
+    s.posK = b.take<int>(nt);
+    s.posD = b.take<int>(nt);
+    s.nrot = b.take<int>(nt);
+    s.kcnt = b.take<int>(nt);
+  } else {
+    s.Z = s.Qtmp = s.U = nullptr;
+  }
+}
+
+// Lays out (or sizes, when base is null) the workspace of one solve.
+size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
+  Bump b{base};
+  slab_layout(b, n, nb, vectors, p);
+  stedc_layout(b, n, 1, n, vectors, &p->sw);
   return b.off + 256;
 }
 
@@ -231,7 +304,13 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
   if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
   ctx->info_dev = vectors ? p.sw.info : nullptr;
-  if ((rc = zq::stedc(n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
+  {
+    const int zero = 0;
+    p.sw.stage = &ctx->stage;
+    if ((rc = zq::stage_begin(ctx->stage))) return rc;
+    if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
+    if ((rc = zq::stage_end(ctx->stage, s))) return rc;
+  }
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   if (vectors) {
     if ((rc = zq::backtransform(n, nb, p.M, n, p.rw.Tf, p.e, p.sw.Z, n, p.X, n, p.bw, s))) return rc;
@@ -243,6 +322,187 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[5], s));
   ctx->have_times = true;
   return ZQ_OK;
+}
+
+int check_args(int n2, const void* D, int nld2, const void* eig);
+
+// ---------------------------------------------------------------------------
+// Batches: problems grouped by panel width, each group solved in lockstep
+// (sizes descending), every problem's buffers in a slab of fixed stride.
+// ---------------------------------------------------------------------------
+struct BatchPlan {
+  Plan slab;          // problem 0's buffers
+  size_t zs = 0;      // slab stride (bytes)
+  int* nvec = nullptr;
+  int* zrow = nullptr;
+  zq::BatchProb* probs = nullptr;
+  double* dall = nullptr;
+  double* eall = nullptr;
+  double* wall = nullptr;
+  zq::StedcWork sw;
+  zq::QGemmTask* rtasks = nullptr;
+  zq::QGemmTask* btasks = nullptr;
+  zq::SumTask* sums = nullptr;
+};
+
+size_t batch_layout(int nprob, const int* nh, int nb, bool vectors, char* base, BatchPlan* bp) {
+  const int L = nh[0];
+  int ntot = 0;
+  for (int z = 0; z < nprob; ++z) ntot += nh[z];
+  Bump sizing{nullptr};
+  Plan tmp;
+  slab_layout(sizing, L, nb, vectors, &tmp);
+  const size_t zs = (sizing.off + 255) & ~size_t(255);
+  Bump b{base};
+  char* slabs = b.take<char>(zs * nprob);
+  Bump s0{slabs};
+  slab_layout(s0, L, nb, vectors, &bp->slab);
+  bp->zs = zs;
+  bp->nvec = b.take<int>(nprob);
+  bp->zrow = b.take<int>(nprob);
+  bp->probs = b.take<zq::BatchProb>(nprob);
+  bp->dall = b.take<double>(ntot + 1);
+  bp->eall = b.take<double>(ntot + 1);
+  bp->wall = b.take<double>(ntot + 1);
+  stedc_layout(b, ntot, nprob, L, vectors, &bp->sw);
+  bp->rtasks = b.take<zq::QGemmTask>(zq::reduce_batched_tasks(nprob, nh, nb));
+  if (vectors) {
+    bp->btasks = b.take<zq::QGemmTask>(zq::backtransform_batched_tasks(nprob, nh, nb, kNSplit));
+    bp->sums = b.take<zq::SumTask>((size_t)nprob * (L > 1 ? cdiv(L - 1, nb) : 0) + 1);
+  }
+  return b.off + 256;
+}
+
+// One lockstep group on the context's workspace (already large enough).
+// P: the group's problems, sizes descending, P[z].off the forest offsets.
+int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vectors, int* info_out) {
+  hipStream_t s = ctx->stream;
+  const int nprob = (int)P.size();
+  std::vector<int> nh(nprob), zrow(nprob);
+  int ntot = 0;
+  for (int z = 0; z < nprob; ++z) {
+    nh[z] = P[z].n;
+    zrow[z] = ntot;
+    P[z].off = ntot;
+    ntot += P[z].n;
+  }
+  const int L = nh[0];
+  BatchPlan bp;
+  batch_layout(nprob, nh.data(), nb, vectors, static_cast<char*>(ctx->ws), &bp);
+  zq::Staging& st = ctx->stage;
+  int rc;
+  if ((rc = zq::stage_upload(st, nh.data(), nprob * sizeof(int), bp.nvec, s))) return rc;
+  if ((rc = zq::stage_upload(st, zrow.data(), nprob * sizeof(int), bp.zrow, s))) return rc;
+  if ((rc = zq::stage_upload(st, P.data(), nprob * sizeof(zq::BatchProb), bp.probs, s))) return rc;
+  const zq::ZB zb{bp.nvec, bp.zs};
+  if ((rc = zq::load_left_batched(nprob, L, bp.probs, bp.slab.M, L, zb, s))) return rc;
+  if ((rc = zq::reduce_batched(nprob, nh.data(), zb, bp.slab.M, L, nb, bp.slab.d, bp.slab.e, bp.slab.rw, bp.rtasks,
+                               st, s)))
+    return rc;
+  if ((rc = zq::gather_tridiag_batched(nprob, L, bp.probs, bp.slab.d, bp.slab.e, zb, bp.dall, bp.eall, s))) return rc;
+  bp.sw.stage = &st;
+  if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(bp.sw.info, 0, nprob * sizeof(int), s));
+  if ((rc = zq::stedc(nprob, zrow.data(), nh.data(), ntot, L, bp.dall, bp.eall, bp.wall, bp.sw, vectors, s)))
+    return rc;
+  if ((rc = zq::scatter_eig_batched(nprob, L, bp.probs, bp.wall, s))) return rc;
+  if (vectors) {
+    if ((rc = zq::backtransform_batched(nprob, nh.data(), zb, zrow.data(), bp.zrow, nb, bp.slab.M, L,
+                                        bp.slab.rw.Tf, bp.slab.e, bp.sw.Z, ntot, bp.slab.X, L, bp.slab.bw,
+                                        bp.btasks, bp.sums, st, s)))
+      return rc;
+    if ((rc = zq::store_vectors_batched(nprob, L, bp.probs, bp.slab.X, L, zb, s))) return rc;
+    if (info_out) {
+      ZQ_HIP_CHECK(hipMemcpyAsync(info_out, bp.sw.info, nprob * sizeof(int), hipMemcpyDeviceToHost, s));
+      ZQ_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  } else if (info_out) {
+    for (int z = 0; z < nprob; ++z) info_out[z] = 0;
+  }
+  return ZQ_OK;
+}
+
+// Device-memory batch: validates, groups, solves.  info (host, optional)
+// receives each problem's status; with info the call waits for completion.
+int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                  bool vectors, int* info) {
+  std::vector<std::vector<zq::BatchProb>> groups;
+  std::vector<std::vector<int>> gidx;
+  std::vector<int> gnb;
+  for (int i = 0; i < count; ++i) {
+    if (info) info[i] = 0;
+    if (n2[i] == 0) continue;
+    const int n = n2[i] / 2;
+    const int nb = panel_width(ctx, n);
+    size_t g = 0;
+    while (g < gnb.size() && gnb[g] != nb) ++g;
+    if (g == gnb.size()) {
+      gnb.push_back(nb);
+      groups.emplace_back();
+      gidx.emplace_back();
+    }
+    groups[g].push_back(zq::BatchProb{D[i], eig[i], ld2[i], n, 0, 0});
+    gidx[g].push_back(i);
+  }
+  if (groups.empty()) return ZQ_OK;
+  // sizes descending, ties by position (stable)
+  for (size_t g = 0; g < groups.size(); ++g) {
+    std::vector<int> ord(groups[g].size());
+    for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int)k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return groups[g][a].n > groups[g][b].n; });
+    std::vector<zq::BatchProb> P(ord.size());
+    std::vector<int> I(ord.size());
+    for (size_t k = 0; k < ord.size(); ++k) {
+      P[k] = groups[g][ord[k]];
+      I[k] = gidx[g][ord[k]];
+    }
+    groups[g].swap(P);
+    gidx[g].swap(I);
+  }
+  size_t need = 0;
+  for (size_t g = 0; g < groups.size(); ++g) {
+    std::vector<int> nh(groups[g].size());
+    for (size_t k = 0; k < nh.size(); ++k) nh[k] = groups[g][k].n;
+    BatchPlan bp;
+    need = std::max(need, batch_layout((int)nh.size(), nh.data(), gnb[g], vectors, nullptr, &bp));
+  }
+  int rc = ensure(ctx->ws, ctx->ws_bytes, need);
+  if (rc) return rc;
+  ctx->have_times = false;
+  ctx->info_dev = nullptr;
+  if ((rc = zq::stage_begin(ctx->stage))) return rc;
+  int status = ZQ_OK;
+  for (size_t g = 0; g < groups.size() && !rc; ++g) {
+    std::vector<int> ginfo(info ? groups[g].size() : 0);
+    rc = solve_group(ctx, groups[g], gnb[g], vectors, info ? ginfo.data() : nullptr);
+    if (!rc && info) {
+      for (size_t k = 0; k < ginfo.size(); ++k) {
+        info[gidx[g][k]] = ginfo[k];
+        if (ginfo[k] != 0 && status == ZQ_OK) status = 1;
+      }
+    }
+  }
+  const int rc2 = zq::stage_end(ctx->stage, ctx->stream);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  return status;
+}
+
+int check_batch(const zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                int* info) {
+  if (!c) return ZQ_ERR_ARG;
+  if (count < 0) return -2;
+  if (count == 0) return ZQ_OK;
+  if (!n2) return -3;
+  if (!D) return -4;
+  if (!ld2) return -5;
+  if (!eig) return -6;
+  int bad = 0;
+  for (int i = 0; i < count; ++i) {
+    const int r = check_args(n2[i], D[i], ld2[i], eig[i]);
+    if (info) info[i] = r;
+    if (r) bad = 1;
+  }
+  return bad ? -8 : ZQ_OK;
 }
 
 int check_args(int n2, const void* D, int nld2, const void* eig) {
@@ -307,6 +567,7 @@ int zq_destroy(zq_context* c) {
   if (!c) return ZQ_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  zq::stage_free(c->stage);
   if (c->ws) (void)hipFree(c->ws);
   if (c->dD) (void)hipFree(c->dD);
   if (c->deig) (void)hipFree(c->deig);
@@ -369,6 +630,58 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
   return info;
 }
 
+int zq_zquatev_batched_device(zq_context* c, int count, const int* n2, double* const* D, const int* ld2,
+                              double* const* eig, int jobz, int* info) {
+  int rc = check_batch(c, count, n2, D, ld2, eig, info);
+  if (rc) return rc;
+  if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -7;
+  if (count == 0) return ZQ_OK;
+  if ((rc = with_device(c))) return rc;
+  return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', info);
+}
+
+int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                       int* info) {
+  int rc = check_batch(c, count, n2, D, ld2, eig, info);
+  if (rc) return rc;
+  if (count == 0) return ZQ_OK;
+  if ((rc = with_device(c))) return rc;
+  // device copies, each problem tight (ld = n2), one allocation
+  std::vector<size_t> off(count + 1, 0), eoff(count + 1, 0);
+  for (int i = 0; i < count; ++i) {
+    off[i + 1] = off[i] + (((size_t)n2[i] * n2[i] * 16 + 255) & ~size_t(255));
+    eoff[i + 1] = eoff[i] + (size_t)(n2[i] / 2);
+  }
+  if ((rc = ensure(c->dD, c->dD_bytes, off[count] + 256))) return rc;
+  if ((rc = ensure(reinterpret_cast<void*&>(c->deig), c->deig_bytes, (eoff[count] + 1) * sizeof(double)))) return rc;
+  hipStream_t s = c->stream;
+  std::vector<double*> dptr(count), eptr(count);
+  std::vector<int> dld(count);
+  for (int i = 0; i < count; ++i) {
+    dptr[i] = reinterpret_cast<double*>(static_cast<char*>(c->dD) + off[i]);
+    eptr[i] = c->deig + eoff[i];
+    dld[i] = n2[i] > 0 ? n2[i] : 1;
+    const int n = n2[i] / 2;
+    if (n2[i] > 0)
+      ZQ_HIP_CHECK(hipMemcpy2DAsync(dptr[i], (size_t)n2[i] * 16, D[i], (size_t)ld2[i] * 16, (size_t)n2[i] * 16, n,
+                                    hipMemcpyHostToDevice, s));
+  }
+  std::vector<int> inf(count, 0);
+  rc = solve_batched(c, count, n2, dptr.data(), dld.data(), eptr.data(), true, inf.data());
+  if (rc < 0) return rc;
+  for (int i = 0; i < count; ++i) {
+    if (n2[i] == 0) continue;
+    const int n = n2[i] / 2;
+    ZQ_HIP_CHECK(hipMemcpy2DAsync(D[i], (size_t)ld2[i] * 16, dptr[i], (size_t)n2[i] * 16, (size_t)n2[i] * 16, n2[i],
+                                  hipMemcpyDeviceToHost, s));
+    ZQ_HIP_CHECK(hipMemcpyAsync(eig[i], eptr[i], (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  ZQ_HIP_CHECK(hipStreamSynchronize(s));
+  if (info)
+    for (int i = 0; i < count; ++i) info[i] = inf[i];
+  return rc;
+}
+
 int zq_zquatev(int n2, double* D, int nld2, double* eig) {
   int rc = check_args(n2, D, nld2, eig);
   if (rc || n2 == 0) return rc;
@@ -413,7 +726,11 @@ int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, dou
   if ((rc = make_plan(c, n, 32, vec, &p))) return rc;
   hipStream_t s = c->stream;
   if (vec) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
-  if ((rc = zq::stedc(n, dd, de, dw, p.sw, vec, s))) return rc;
+  const int zero = 0;
+  p.sw.stage = &c->stage;
+  if ((rc = zq::stage_begin(c->stage))) return rc;
+  if ((rc = zq::stedc(1, &zero, &n, n, n, dd, de, dw, p.sw, vec, s))) return rc;
+  if ((rc = zq::stage_end(c->stage, s))) return rc;
   if (vec) ZQ_HIP_CHECK(hipMemcpyAsync(dZ, p.sw.Z, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
   return ZQ_OK;
 }

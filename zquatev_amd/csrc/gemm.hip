@@ -58,20 +58,12 @@ __device__ __forceinline__ quat load_opB(const QOperand& B, int k, int jcol, int
   return B.conj_trans ? qconj(v) : v;
 }
 
-// C = beta C + alpha A B over quaternions.  gridDim.z > 1 splits K into
-// gridDim.z chunks of kchunk; chunk z writes C + z * cstride with beta = 0.
-__global__ void __launch_bounds__(256) k_qgemm(int m, int nn, int K, double alpha, QOperand A, QOperand B, double beta,
-                                               quat* __restrict__ C, int ldc, int lower_only, int kchunk,
-                                               size_t cstride) {
-  const int row0 = blockIdx.x * QBM;
-  const int col0 = blockIdx.y * QBN;
-  if (lower_only && row0 + QBM - 1 < col0) return;
-  int kbeg = 0, kend = K;
-  if (gridDim.z > 1) {
-    kbeg = blockIdx.z * kchunk;
-    kend = min(K, kbeg + kchunk);
-    C += (size_t)blockIdx.z * cstride;
-  }
+// One tile (QBM x QBN quaternions at row0, col0) of C = beta C + alpha A B
+// over logical k in [kbeg, kend).
+__device__ __forceinline__ void qgemm_tile(const QGemmTask& t, int row0, int col0, int kbeg, int kend, quat* C) {
+  const int m = t.m, nn = t.nn, K = t.K;
+  const QOperand& A = t.A;
+  const QOperand& B = t.B;
   __shared__ double As[QBK][QBM][4];
   __shared__ double Bs[QBK][QBN][4];
   const int tid = threadIdx.x;
@@ -131,14 +123,15 @@ __global__ void __launch_bounds__(256) k_qgemm(int m, int nn, int K, double alph
     __syncthreads();
   }
   // epilogue: lane holds quaternion (row wr*16 + 4a + lane/16, col wc*32 + 16b + lane%16)
+  const double alpha = t.alpha, beta = t.beta;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int i = row0 + wr * 16 + 4 * a + (lane >> 4);
       const int jj = col0 + wc * 32 + 16 * b + (lane & 15);
-      if (i < m && jj < nn && (!lower_only || i >= jj)) {
-        quat* cp = C + i + (size_t)jj * ldc;
+      if (i < m && jj < nn && (!t.lower_only || i >= jj)) {
+        quat* cp = C + i + (size_t)jj * t.ldc;
         quat r = quat{alpha * acc[a][b][0], alpha * acc[a][b][1], alpha * acc[a][b][2], alpha * acc[a][b][3]};
         if (beta != 0.0) r = qadd(r, qscale(qload(cp), beta));
         qstore(cp, r);
@@ -147,11 +140,46 @@ __global__ void __launch_bounds__(256) k_qgemm(int m, int nn, int K, double alph
   }
 }
 
+// One task per launch.  gridDim.z > 1 splits [t.kbeg, t.kend) into chunks of
+// kchunk; chunk z writes C + z * cstride (the task then has beta = 0).
+__global__ void __launch_bounds__(256) k_qgemm(QGemmTask t, int kchunk, size_t cstride) {
+  const int row0 = blockIdx.x * QBM;
+  const int col0 = blockIdx.y * QBN;
+  if (t.lower_only && row0 + QBM - 1 < col0) return;
+  int kbeg = t.kbeg, kend = t.kend;
+  quat* C = t.C;
+  if (gridDim.z > 1) {
+    kbeg = t.kbeg + blockIdx.z * kchunk;
+    kend = min(t.kend, kbeg + kchunk);
+    C += (size_t)blockIdx.z * cstride;
+  }
+  qgemm_tile(t, row0, col0, kbeg, kend, C);
+}
+
+// Many independent tasks per launch (a batch of problems, or the K chunks of
+// a split product listed as tasks): task blockIdx.z.
+__global__ void __launch_bounds__(256) k_qgemm_batch(const QGemmTask* __restrict__ tasks) {
+  const QGemmTask t = tasks[blockIdx.z];
+  const int row0 = blockIdx.x * QBM;
+  const int col0 = blockIdx.y * QBN;
+  if (row0 >= t.m || col0 >= t.nn) return;
+  if (t.lower_only && row0 + QBM - 1 < col0) return;
+  qgemm_tile(t, row0, col0, t.kbeg, t.kend, t.C);
+}
+
 int qgemm(int m, int nn, int k, double alpha, const QOperand& A, const QOperand& B, double beta, quat* C, int ldc,
           bool lower_only, hipStream_t s) {
   if (m <= 0 || nn <= 0) return ZQI_OK;
   dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, 1);
-  launch_k(k_qgemm, grid, dim3(256), 0, s, m, nn, k, alpha, A, B, beta, C, ldc, lower_only ? 1 : 0, k, 0);
+  QGemmTask t{m, nn, k, 0, k, ldc, lower_only ? 1 : 0, alpha, beta, A, B, C};
+  launch_k(k_qgemm, grid, dim3(256), 0, s, t, k, (size_t)0);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, hipStream_t s) {
+  if (ntasks <= 0 || maxm <= 0 || maxnn <= 0) return ZQI_OK;
+  dim3 grid((maxm + QBM - 1) / QBM, (maxnn + QBN - 1) / QBN, ntasks);
+  launch_k(k_qgemm_batch, grid, dim3(256), 0, s, tasks_dev);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -172,24 +200,34 @@ int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int 
   const int nz = splitk_count(K, nsplit, &kchunk);
   QOperand Bo{B, B, ldb, ldb, INT_MAX, false, INT_MIN / 2};
   dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, nz);
-  launch_k(k_qgemm, grid, dim3(256), 0, s, m, nn, K, 1.0, A, Bo, 0.0, Ypart, m, 0, kchunk,
-                     (size_t)m * nn);
+  QGemmTask t{m, nn, K, 0, K, m, 0, 1.0, 0.0, A, Bo, Ypart};
+  launch_k(k_qgemm, grid, dim3(256), 0, s, t, kchunk, (size_t)m * nn);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
 // ---------------------------------------------------------------------------
-// Real FP64 GEMM for the divide & conquer merges: C[:, cmap[j]] = A (m x K) B (K x K),
-// K read from the device (data-dependent after deflation).  Batched over merges
-// through gridDim.z with per-merge offsets.
+// Real FP64 GEMM for the divide & conquer merges of one tree level, one merge
+// per blockIdx.z: for the merge of rows [lo, hi) (ns = hi - lo) of problem
+// base ob,  Z[lo:hi, lo - ob + posK[j]] = Qtmp_m (ns x K) U_m (K x K), with
+// K = kcnt[z] read on the device (data-dependent after deflation) and the
+// merge's Qtmp / U blocks at lo * lq (ld ns).
 // ---------------------------------------------------------------------------
 constexpr int DBM = 64, DBN = 64, DBK = 16;
 
-__global__ void __launch_bounds__(256) k_dgemm_nn(int m, const int* __restrict__ kdev, const double* __restrict__ A,
-                                                  int lda, const double* __restrict__ B, int ldb,
-                                                  double* __restrict__ C, int ldc, const int* __restrict__ cmap) {
-  const int K = *kdev;
+__global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict__ descs, const int* __restrict__ kcnt,
+                                                     const double* __restrict__ Qtmp, const double* __restrict__ U,
+                                                     int lq, double* __restrict__ Z, int ldz,
+                                                     const int* __restrict__ posK) {
+  const MergeDesc md = descs[blockIdx.z];
+  const int K = kcnt[blockIdx.z];
+  const int m = md.hi - md.lo;
   const int row0 = blockIdx.x * DBM, col0 = blockIdx.y * DBN;
   if (col0 >= K || row0 >= m) return;
+  const double* A = Qtmp + (size_t)md.lo * lq;
+  const double* B = U + (size_t)md.lo * lq;
+  const int lda = m, ldb = m;
+  double* C = Z + md.lo + (size_t)(md.lo - md.ob) * ldz;
+  const int* cmap = posK + md.lo;
   __shared__ double As[DBK][DBM + 1];
   __shared__ double Bs[DBK][DBN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -231,20 +269,20 @@ __global__ void __launch_bounds__(256) k_dgemm_nn(int m, const int* __restrict__
     for (int b = 0; b < 2; ++b) {
       const int j = col0 + wc * 32 + 16 * b + (lane & 15);
       if (j >= K) continue;
-      const int cj = cmap ? cmap[j] : j;
+      const int cj = cmap[j];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = row0 + wr * 32 + 16 * a + 4 * r + (lane >> 4);
-        if (i < m) C[i + (size_t)cj * ldc] = acc[a][b][r];
+        if (i < m) C[i + (size_t)cj * ldz] = acc[a][b][r];
       }
     }
 }
 
-int dgemm_nn(int m, int, const int* k_dev, int k_max, const double* A, int lda, const double* B, int ldb, double* C,
-             int ldc, const int* cmap, hipStream_t s) {
-  if (m <= 0 || k_max <= 0) return ZQI_OK;
-  dim3 grid((m + DBM - 1) / DBM, (k_max + DBN - 1) / DBN, 1);
-  launch_k(k_dgemm_nn, grid, dim3(256), 0, s, m, k_dev, A, lda, B, ldb, C, ldc, cmap);
+int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const double* Qtmp,
+                 const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s) {
+  if (nmerge <= 0 || nsmax <= 0) return ZQI_OK;
+  dim3 grid((nsmax + DBM - 1) / DBM, (nsmax + DBN - 1) / DBN, nmerge);
+  launch_k(k_dgemm_merge, grid, dim3(256), 0, s, descs_dev, kcnt, Qtmp, U, lq, Z, ldz, posK);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

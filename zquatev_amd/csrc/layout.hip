@@ -39,6 +39,77 @@ __global__ void k_offdiag_abs(int n, const quat* __restrict__ e, double* __restr
   if (i < n - 1) ea[i] = sqrt(qnorm2(qload(e + i)));
 }
 
+// ---- batches: problem blockIdx.z, its table entry P[z], its buffers by zb ----
+__global__ void __launch_bounds__(256) k_load_left_b(const BatchProb* __restrict__ P, quat* __restrict__ M, int ld,
+                                                     ZB zb) {
+  const int n = P[blockIdx.z].n;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || j >= n || i < j) return;
+  const int ld2 = P[blockIdx.z].ld2;
+  const double2* col = reinterpret_cast<const double2*>(P[blockIdx.z].D) + (size_t)j * ld2;
+  const double2 a = col[i];
+  const double2 b = col[n + i];
+  qstore(zoff(M, zb) + i + (size_t)j * ld, quat{a.x, a.y, b.x, b.y});
+}
+
+__global__ void __launch_bounds__(256) k_store_vec_b(const BatchProb* __restrict__ P, const quat* __restrict__ X,
+                                                     int ldx, ZB zb) {
+  const int n = P[blockIdx.z].n;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || j >= n) return;
+  const int ld2 = P[blockIdx.z].ld2;
+  const quat x = qload(zoff(X, zb) + i + (size_t)j * ldx);
+  double2* left = reinterpret_cast<double2*>(P[blockIdx.z].D) + (size_t)j * ld2;
+  double2* right = reinterpret_cast<double2*>(P[blockIdx.z].D) + (size_t)(j + n) * ld2;
+  left[i] = make_double2(x.ar, x.ai);
+  left[n + i] = make_double2(x.br, x.bi);
+  right[i] = make_double2(-x.br, x.bi);
+  right[n + i] = make_double2(x.ar, -x.ai);
+}
+
+__global__ void __launch_bounds__(256) k_gather_tri_b(const BatchProb* __restrict__ P, const double* __restrict__ d,
+                                                      const quat* __restrict__ e, ZB zb, double* __restrict__ dall,
+                                                      double* __restrict__ eall) {
+  const int n = P[blockIdx.z].n, off = P[blockIdx.z].off;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  dall[off + i] = zoff(d, zb)[i];
+  if (i < n - 1) eall[off + i] = sqrt(qnorm2(qload(zoff(e, zb) + i)));
+}
+
+__global__ void __launch_bounds__(256) k_scatter_eig_b(const BatchProb* __restrict__ P, const double* __restrict__ wall) {
+  const int n = P[blockIdx.z].n, off = P[blockIdx.z].off;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) P[blockIdx.z].eig[i] = wall[off + i];
+}
+
+int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s) {
+  if (nprob <= 0 || L <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_load_left_b, dim3((L + 255) / 256, L, nprob), dim3(256), 0, s, P, M, ld, zb);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+int store_vectors_batched(int nprob, int L, const BatchProb* P, const quat* X, int ldx, ZB zb, hipStream_t s) {
+  if (nprob <= 0 || L <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_store_vec_b, dim3((L + 255) / 256, L, nprob), dim3(256), 0, s, P, X, ldx, zb);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+int gather_tridiag_batched(int nprob, int L, const BatchProb* P, const double* d, const quat* e, ZB zb, double* dall,
+                           double* eall, hipStream_t s) {
+  if (nprob <= 0 || L <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_gather_tri_b, dim3((L + 255) / 256, 1, nprob), dim3(256), 0, s, P, d, e, zb, dall, eall);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+int scatter_eig_batched(int nprob, int L, const BatchProb* P, const double* wall, hipStream_t s) {
+  if (nprob <= 0 || L <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_scatter_eig_b, dim3((L + 255) / 256, 1, nprob), dim3(256), 0, s, P, wall);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
 int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s) {
   hipLaunchKernelGGL(k_load_left, dim3((n + 255) / 256, n), dim3(256), 0, s, n, D, ld2, M, ld);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;

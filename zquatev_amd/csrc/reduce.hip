@@ -21,6 +21,7 @@
 #include "zq_internal.h"
 #include <climits>
 #include <algorithm>
+#include <vector>
 
 namespace zq {
 
@@ -49,15 +50,24 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // Also finishes w_{j-1} = p_{j-1} - K u_{c-1} (K = u^*p / 2 from the previous
 // column's partials): each W(i, j-1), i > c, is read and rewritten by the one
 // thread that uses it; W(c, j-1) is formed on the fly (no later kernel reads
-// it).  Zeroes the accumulator TS that the matvec of column c fills.
+// it).  Finally the block's 16 rows of the tall-skinny products W^* x and
+// V^* x (x the updated column, rows >= c+2) go to TSP[block] (the matvec
+// kernel adds them up in a fixed order).
 // 16 threads per row, KPT panel columns each; every load is issued up front
 // at a clamped address.
 template <int KPT>
 __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                                    quat* __restrict__ W, int ldw, const double* __restrict__ KP,
-                                                   int nkp, double* __restrict__ part, quat* __restrict__ TS,
-                                                   int nb) {
+                                                   int nkp, double* __restrict__ part, double* __restrict__ TSP,
+                                                   int ldtsp, ZB zb) {
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    if (c + (int)blockIdx.x * RQ >= n) return;
+    nkp = (n - c + RQ - 1) / RQ;
+    M = zoff(M, zb); W = zoff(W, zb); KP = zoff(KP, zb); part = zoff(part, zb); TSP = zoff(TSP, zb);
+  }
   __shared__ quat cw[NBMAX], cv[NBMAX];
+  __shared__ quat xs[RQ];
   __shared__ quat red[16][RQ];
   __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
@@ -82,10 +92,6 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
     cvl = qload(M + c + (size_t)(p + tid) * ld);
     if (tid == j - 1) ucc = qload(M + c + (size_t)(c - 1) * ld);
   }
-  // the matvec of this column accumulates into TS
-  if (blockIdx.x == 0) {
-    for (int k = tid; k < 2 * nb; k += RB) qstore(TS + k, qzero());
-  }
   const double K = (j > 0) ? 0.5 * block_sum(t, sh) : 0.0;
   if (tid < j) {
     if (tid == j - 1) cwl = qsub(cwl, qscale(ucc, K));
@@ -102,6 +108,7 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
       if (l == j - 1) {
         wv = qsub(wv, qscale(uprev, K));
         if (i > c && i < n) qstore(W + i + (size_t)l * ldw, wv);
+        ww[k] = wv;
       }
       qfma(acc, vv[k], cw[l]);
       qfma(acc, wv, cv[l]);
@@ -110,6 +117,7 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
   red[q][r] = acc;
   __syncthreads();
   double ss = 0.0;
+  quat xm = qzero();
   if (q == 0 && i < n) {
     quat m = mcur;
     if (j > 0) {
@@ -119,10 +127,40 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
       m = qsub(m, sum);
       qstore(M + i + (size_t)c * ld, m);
     }
-    if (i >= c + 2) ss = qnorm2(m);
+    if (i >= c + 2) {
+      ss = qnorm2(m);
+      xm = m;
+    }
   }
-  ss = block_sum(ss, sh);
+  if (q == 0) xs[r] = xm;
+  ss = block_sum(ss, sh);   // its barriers also publish xs
   if (tid == 0) part[blockIdx.x] = ss;
+  if (j > 0) {
+    // this thread's panel columns l = q + 16 k against x_i of its row, then
+    // summed over the 16 rows (the 16 lanes sharing q) by a halving butterfly
+    constexpr int NV = (8 * KPT < 16) ? 16 : 8 * KPT;
+    double v[NV];
+#pragma unroll
+    for (int t = 0; t < NV; ++t) v[t] = 0.0;
+    const quat x = xs[r];
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      if (q + 16 * k < j) {
+        quat a = qzero(), b = qzero();
+        qfma_cl(a, ww[k], x);
+        qfma_cl(b, vv[k], x);
+        v[8 * k + 0] = a.ar; v[8 * k + 1] = a.ai; v[8 * k + 2] = a.br; v[8 * k + 3] = a.bi;
+        v[8 * k + 4] = b.ar; v[8 * k + 5] = b.ai; v[8 * k + 6] = b.br; v[8 * k + 7] = b.bi;
+      }
+    }
+    row16_halving<NV>(v);
+#pragma unroll
+    for (int t = 0; t < NV / 16; ++t) {
+      const int idx = r * (NV / 16) + t;
+      const int k = idx >> 3, l = q + 16 * k;
+      if (k < KPT && l < j) TSP[(size_t)blockIdx.x * ldtsp + 8 * l + (idx & 7)] = v[t];
+    }
+  }
 }
 
 // Reflector for column c: x = M[c+1:, c].  H x = alpha e_1 with
@@ -167,18 +205,30 @@ __device__ __forceinline__ quat u_from(const quat& x, int c, int i, const Refl& 
 //   y(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
 // every (slot, row) written exactly once, so the sum is deterministic.
 // Blocks are enumerated row-group major: group g has 2 (g + 1) strips.
-// Second phase: the tall-skinny products W^* u and V^* u over the block's
-// rows, groups of 8 panel columns spread over the row group's blocks, added
-// into TS = [W^* u | V^* u] with FP64 atomics.
+// The ceil(8 j / 64) blocks after the G (G + 1) tiles finish the tall-skinny
+// products TS = [W^* u | V^* u]: they add the per-block partials of W^* x and
+// V^* x that k_col_update left in TSP (8 waves over the partial index, then
+// the 8 wave sums, in a fixed order), scale by the reflector's gamma and add
+// the row c+1 term, u_{c+1} = u1.
 __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                               const quat* __restrict__ W, int ldw, const double* __restrict__ part,
                                               int nparts, double* __restrict__ scal, double* __restrict__ d,
                                               quat* __restrict__ e, quat* __restrict__ Prow, quat* __restrict__ Pcol,
-                                              int ldp, quat* __restrict__ ucur, quat* __restrict__ TS, int nb) {
+                                              int ldp, quat* __restrict__ ucur, quat* __restrict__ TS, int nb,
+                                              const double* __restrict__ TSP, int ldtsp, ZB zb) {
+  const int NT = j > 0 ? (8 * j + 63) / 64 : 0;
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    nparts = (n - c + RQ - 1) / RQ;
+    const int Gz = (n - c - 1 + SG - 1) / SG;
+    if ((int)blockIdx.x >= Gz * (Gz + 1) + NT) return;
+    M = zoff(M, zb); W = zoff(W, zb); part = zoff(part, zb); scal = zoff(scal, zb); d = zoff(d, zb);
+    e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); ucur = zoff(ucur, zb); TS = zoff(TS, zb);
+    TSP = zoff(TSP, zb);
+  }
   __shared__ double sh[SYT / 64];
   __shared__ quat uJ[SW];
   __shared__ quat red[8][SG];
-  __shared__ quat tsr[8][8][2];
   const int s0 = c + 1;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -201,14 +251,16 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   const quat xa = qload(M + rac + (size_t)c * ld);
   const quat xj = qload(M + min(C0 + (tid & (SW - 1)), n - 1) + (size_t)c * ld);
   const quat x1 = qload(M + (c + 1) + (size_t)c * ld);
-  // tall-skinny task of this block (at most one): rows of group tg, panel columns 8 tl .. 8 tl + 7
-  const int ngr = (j + 7) >> 3;
+  // tall-skinny finishing block: partial sums of entries 64 bt + lane, wave w taking every 8th block
   const int Gt = (n - s0 + SG - 1) / SG;
-  const bool has_ts = b < Gt * ngr;
-  const int tg = has_ts ? b / ngr : 0, tl = has_ts ? b - (b / ngr) * ngr : 0;
-  const int trow = s0 + SG * tg + (tid >> 3);
-  const int tcol = 8 * tl + (tid & 7);
-  const int trc = min(trow, n - 1), tcc = min(tcol, j > 0 ? j - 1 : 0);
+  const int bt = b - Gt * (Gt + 1);
+  const bool has_ts = bt >= 0 && bt < NT;
+  double tsacc = 0.0;
+  if (has_ts) {
+    const int eidx = 64 * bt + lane;
+#pragma unroll 8
+    for (int k = w; k < nparts; k += 8) tsacc += TSP[(size_t)k * ldtsp + eidx];
+  }
   double tail = 0.0;
   for (int k = tid; k < nparts; k += SYT) tail += part[k];
   tail = block_sum(tail, sh);
@@ -256,37 +308,28 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
-  // tall-skinny task: W^* u and V^* u over 64 rows for 8 panel columns
   if (has_ts) {
-    const quat tw = qload(W + trc + (size_t)tcc * ldw);
-    const quat tv = qload(M + trc + (size_t)(p + tcc) * ld);
-    const quat tx = qload(M + trc + (size_t)c * ld);
-    const bool ok = trow < n && tcol < j;
-    quat aw = qzero(), av = qzero();
-    if (ok) {
-      const quat ur = u_from(tx, c, trow, R);
-      qfma_cl(aw, tw, ur);
-      qfma_cl(av, tv, ur);
-    }
-    // sum over the 8 rows of the wave (lanes differing in bits 3..5)
-    double t8[8] = {aw.ar, aw.ai, aw.br, aw.bi, av.ar, av.ai, av.br, av.bi};
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double x = t8[k] + xlane<8>(t8[k]);
-      x = bf16(x, x);
-      t8[k] = bf32(x, x);
-    }
-    if (lane < 8) {
-      tsr[w][lane][0] = quat{t8[0], t8[1], t8[2], t8[3]};
-      tsr[w][lane][1] = quat{t8[4], t8[5], t8[6], t8[7]};
-    }
+    __shared__ double tsr[8][64];
+    tsr[w][lane] = tsacc;
     __syncthreads();
-    if (tid < 64) {
-      const int side = tid >> 5, ll = (tid >> 2) & 7, comp = tid & 3;
-      double acc = 0.0;
+    if (tid < 16) {
+      // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
+      const int e0 = 64 * bt + 4 * tid;
+      const int l = e0 >> 3, side = (e0 >> 2) & 1;
+      if (l < j) {
+        double sc[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc += reinterpret_cast<const double*>(&tsr[k][ll][side])[comp];
-      if (8 * tl + ll < j) atomicAdd(reinterpret_cast<double*>(TS + (size_t)side * nb + 8 * tl + ll) + comp, acc);
+        for (int comp = 0; comp < 4; ++comp) {
+          double t = 0.0;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
+          sc[comp] = t;
+        }
+        const quat rowv = side == 0 ? qload(W + (c + 1) + (size_t)l * ldw) : qload(M + (c + 1) + (size_t)(p + l) * ld);
+        quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
+        qfma_cl(t, rowv, R.u1);
+        qstore(TS + (size_t)side * nb + l, t);
+      }
     }
   }
 }
@@ -299,7 +342,13 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
                                                 quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                                 const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
                                                 int nb, const quat* __restrict__ ucur,
-                                                double* __restrict__ KP, quat* __restrict__ Gm) {
+                                                double* __restrict__ KP, quat* __restrict__ Gm, ZB zb) {
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    if (c + 1 + (int)blockIdx.x * RQ >= n) return;
+    M = zoff(M, zb); W = zoff(W, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb);
+    ucur = zoff(ucur, zb); KP = zoff(KP, zb); Gm = zoff(Gm, zb);
+  }
   __shared__ quat a[NBMAX], b[NBMAX];
   __shared__ quat red[16][RQ];
   __shared__ double sh[RB / 64];
@@ -359,10 +408,18 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
 }
 
 // w = p - (u^* p / 2) u  (last column of a panel; the others are finished by
-// the next column's k_col_update)
+// the next column's k_col_update).  In a batch, only the problems whose panel
+// ends at column c (j = nb - 1, or c = n - 2) take part.
 __global__ void __launch_bounds__(RB) k_wfinal(const quat* __restrict__ M, int ld, int n, int c, int j,
                                                quat* __restrict__ W, int ldw, const double* __restrict__ KP,
-                                               int nkp) {
+                                               int nkp, ZB zb, int nb) {
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    if (!(j == nb - 1 || c == n - 2)) return;
+    if (c + 1 + (int)blockIdx.x * RB >= n) return;
+    nkp = (n - c - 1 + RQ - 1) / RQ;
+    M = zoff(M, zb); W = zoff(W, zb); KP = zoff(KP, zb);
+  }
   __shared__ double sh[RB / 64];
   double t = 0.0;
   for (int i = threadIdx.x; i < nkp; i += RB) t += KP[i];
@@ -380,8 +437,14 @@ __global__ void __launch_bounds__(RB) k_wfinal(const quat* __restrict__ M, int l
 // T = (I + triu(G, 1))^{-1} for each panel: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
 // T[j, j] = 1, zero below (equivalent to the usual compact-WY recurrence
 // since u^* u = 2 makes every tau 1).  One workgroup per panel.
-__global__ void __launch_bounds__(RB) k_tfactor(int n, int nb, const quat* __restrict__ Gall, quat* __restrict__ Tall) {
+__global__ void __launch_bounds__(RB) k_tfactor(int n, int nb, const quat* __restrict__ Gall, quat* __restrict__ Tall,
+                                                ZB zb) {
   const int panel = blockIdx.x;
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    if (panel >= (n - 1 + nb - 1) / nb) return;
+    Gall = zoff(Gall, zb); Tall = zoff(Tall, zb);
+  }
   const int p = panel * nb;
   const int wdt = min(nb, n - 1 - p);
   const quat* G = Gall + (size_t)panel * nb * nb;
@@ -402,7 +465,11 @@ __global__ void __launch_bounds__(RB) k_tfactor(int n, int nb, const quat* __res
   }
 }
 
-__global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* __restrict__ d) {
+__global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* __restrict__ d, ZB zb) {
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    M = zoff(M, zb); d = zoff(d, zb);
+  }
   if (threadIdx.x == 0) d[n - 1] = M[(n - 1) + (size_t)(n - 1) * ld].ar;
 }
 
@@ -438,7 +505,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
   if (n <= 0) return ZQI_OK;
   if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
   if (n == 1) {
-    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d);
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
     return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
   }
   const int ldw = n;
@@ -452,20 +519,23 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       const int nrow = n - c;                      // rows c..n-1
       const int nblk = cdiv(nrow, RQ);
       const int nkp = cdiv(nrow, RQ);              // blocks of the previous column's k_wreduce
-      launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part, w.TS, nb);
+      launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part, w.TSP, w.ldtsp,
+                        ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int nmain = std::max(G * (G + 1), G * ((j + 7) / 8));
+      const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 0);
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
         launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
-                           w.scal, d, e, w.Prow, w.Pcol, n, w.ucur, w.TS, nb);
+                 w.scal, d, e, w.Prow, w.Pcol, n, w.ucur, w.TS, nb, w.TSP, w.ldtsp, ZB{nullptr, 0});
       }
       const int nb2 = cdiv(sz, RQ);
-      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.ucur, w.KP, Gp);
+      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.ucur, w.KP, Gp,
+                     ZB{nullptr, 0});
       if (j == wdt - 1)
-        hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2);
+        hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2,
+                           ZB{nullptr, 0}, nb);
     }
     // trailing update of rows/cols [p+wdt, n): M -= [V W] [W V]^*  (lower triangle)
     const int r0 = p + wdt;
@@ -479,8 +549,82 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       if (rc) return rc;
     }
   }
-  hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d);
-  hipLaunchKernelGGL(k_tfactor, dim3(panel), dim3(RB), 0, s, n, nb, w.Gm, w.Tf);
+  hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
+  hipLaunchKernelGGL(k_tfactor, dim3(panel), dim3(RB), 0, s, n, nb, w.Gm, w.Tf, ZB{nullptr, 0});
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+size_t reduce_batched_tasks(int nprob, const int* nh, int nb) {
+  if (nprob <= 0 || nh[0] < 2) return 1;
+  return (size_t)nprob * cdiv(nh[0] - 1, nb) + 1;
+}
+
+int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, double* d, quat* e, const ReduceWork& w,
+                   QGemmTask* tasks, Staging& st, hipStream_t s) {
+  if (nprob <= 0) return ZQI_OK;
+  if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
+  const int L = nh[0];
+  for (int z = 1; z < nprob; ++z)
+    if (nh[z] > nh[z - 1] || nh[z] < 1) return ZQI_ERR_ARG;
+  const int ldw = ld;
+  const int npan = L > 1 ? cdiv(L - 1, nb) : 0;
+  // trailing-update tasks of every panel, problem by problem: the task of
+  // problem z in panel P is what reduce() launches for z after that panel
+  std::vector<QGemmTask> tt;
+  std::vector<int> toff(npan + 1, 0), tmaxm(npan, 0);
+  for (int P = 0; P < npan; ++P) {
+    const int p = P * nb;
+    toff[P] = (int)tt.size();
+    for (int z = 0; z < nprob && nh[z] - 1 > p; ++z) {
+      const int n = nh[z];
+      const int wdt = std::min(nb, n - 1 - p);
+      const int r0 = p + wdt, mt = n - r0;
+      if (mt <= 0) continue;
+      quat* Mz = reinterpret_cast<quat*>(reinterpret_cast<char*>(M) + (size_t)z * zb.zs);
+      quat* Wz = reinterpret_cast<quat*>(reinterpret_cast<char*>(w.W) + (size_t)z * zb.zs);
+      QOperand A{Mz + r0 + (size_t)p * ld, Wz + r0, ld, ldw, wdt, false, INT_MIN / 2};
+      QOperand B{Wz + r0, Mz + r0 + (size_t)p * ld, ldw, ld, wdt, true, INT_MIN / 2};
+      tt.push_back(QGemmTask{mt, mt, 2 * wdt, 0, 2 * wdt, ld, 1, -1.0, 1.0, A, B, Mz + r0 + (size_t)r0 * ld});
+      tmaxm[P] = std::max(tmaxm[P], mt);
+    }
+  }
+  toff[npan] = (int)tt.size();
+  if (tt.size() > reduce_batched_tasks(nprob, nh, nb)) return ZQI_ERR_ARG;
+  int rc = stage_upload(st, tt.data(), tt.size() * sizeof(QGemmTask), tasks, s);
+  if (rc) return rc;
+  const int kpt = kpt_for(nb);
+  int act = nprob;   // problems with a column c: n - 1 > c (a prefix, sizes descending)
+  for (int c = 0; c < L - 1; ++c) {
+    while (act > 0 && nh[act - 1] - 1 <= c) --act;
+    const int p = (c / nb) * nb, j = c - p;
+    const int nrow = L - c;
+    const int nblk = cdiv(nrow, RQ);
+    launch_col_update(kpt, dim3(nblk, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.KP, nblk, w.part, w.TSP, w.ldtsp,
+                      zb);
+    const int sz = L - c - 1;
+    const int G = cdiv(sz, SG);
+    const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 0);
+    hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
+                       w.scal, d, e, w.Prow, w.Pcol, ld, w.ucur, w.TS, nb, w.TSP, w.ldtsp, zb);
+    const int nb2 = cdiv(sz, RQ);
+    quat* Gp = w.Gm + (size_t)(c / nb) * nb * nb;
+    launch_wreduce(kpt, dim3(nb2, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb, w.ucur,
+                   w.KP, Gp, zb);
+    // w of the last column of a panel: every active problem at a panel end,
+    // else those whose last column is c
+    bool fin = (j == nb - 1);
+    for (int z = 0; z < act && !fin; ++z) fin = (nh[z] - 2 == c);
+    if (fin)
+      hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB), 1, act), dim3(RB), 0, s, M, ld, L, c, j, w.W, ldw, w.KP, nb2,
+                         zb, nb);
+    if (j == nb - 1 || c == L - 2) {
+      const int P = c / nb;
+      const int cnt = toff[P + 1] - toff[P];
+      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], s))) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_last_diag, dim3(1, 1, nprob), dim3(64), 0, s, M, ld, L, d, zb);
+  if (npan > 0) hipLaunchKernelGGL(k_tfactor, dim3(npan, 1, nprob), dim3(RB), 0, s, L, nb, w.Gm, w.Tf, zb);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

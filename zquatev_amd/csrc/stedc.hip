@@ -34,18 +34,17 @@ namespace zq {
 constexpr int LEAF = 64;
 constexpr double DEPS = DBL_EPSILON * 0.5;   // unit roundoff
 
-struct MergeDesc {
-  int lo, mid, hi;
-};
-
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // --------------------------------------------------------------------------
-// scale: s = max(max|d|, max|e|); d, e scaled by 1/s into lam, es
+// scale: s = max(max|d|, max|e|) per problem; d, e scaled by 1/s into lam, es.
+// probs holds (offset, size) per problem, one workgroup each.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_dc_scale(int n, const double* __restrict__ d, const double* __restrict__ e,
-                                                   double* __restrict__ lam, double* __restrict__ es,
-                                                   double* __restrict__ scal) {
+__global__ void __launch_bounds__(1024) k_dc_scale(const int* __restrict__ probs, const double* __restrict__ d,
+                                                   const double* __restrict__ e, double* __restrict__ lam,
+                                                   double* __restrict__ es, double* __restrict__ scal) {
+  const int o = probs[2 * blockIdx.x], n = probs[2 * blockIdx.x + 1];
+  d += o; e += o; lam += o; es += o;
   __shared__ double sh[16];
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(d[i]));
@@ -60,7 +59,7 @@ __global__ void __launch_bounds__(1024) k_dc_scale(int n, const double* __restri
   const double inv = 1.0 / s;
   for (int i = threadIdx.x; i < n; i += 1024) lam[i] = d[i] * inv;
   for (int i = threadIdx.x; i < n - 1; i += 1024) es[i] = fabs(e[i]) * inv;
-  if (threadIdx.x == 0) scal[0] = s;
+  if (threadIdx.x == 0) scal[blockIdx.x] = s;
 }
 
 // subtract the coupling beta = es[s-1] from the diagonal entries beside each split s
@@ -77,14 +76,16 @@ __global__ void k_dc_tear(int nsplit, const int* __restrict__ splits, const doub
 // --------------------------------------------------------------------------
 // leaf: implicit QL with Wilkinson-type shift on a <= 64 tridiagonal.  The
 // scalar recurrence runs identically in every lane; lane k owns row k of the
-// leaf's eigenvector block and applies each rotation to it.
+// leaf's eigenvector block and applies each rotation to it.  A leaf is
+// (lo, hi, problem base, problem index).
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_dc_leaf(int n, const int* __restrict__ leaves, double* __restrict__ lam,
+__global__ void __launch_bounds__(64) k_dc_leaf(const int* __restrict__ leaves, double* __restrict__ lam,
                                                 const double* __restrict__ es, double* __restrict__ Z, int ldz,
                                                 int* __restrict__ info) {
   __shared__ double dd[LEAF], ee[LEAF];
   __shared__ double zz[LEAF][LEAF + 1];
-  const int lo = leaves[2 * blockIdx.x], hi = leaves[2 * blockIdx.x + 1];
+  const int lo = leaves[4 * blockIdx.x], hi = leaves[4 * blockIdx.x + 1];
+  const int ob = leaves[4 * blockIdx.x + 2], pr = leaves[4 * blockIdx.x + 3];
   const int m = hi - lo;
   const int k = threadIdx.x;
   if (k < m) {
@@ -167,9 +168,9 @@ __global__ void __launch_bounds__(64) k_dc_leaf(int n, const int* __restrict__ l
   __syncthreads();
   if (k < m) {
     lam[lo + k] = dd[k];
-    for (int i = 0; i < m; ++i) Z[(lo + k) + (size_t)(lo + i) * ldz] = zz[k][i];
+    for (int i = 0; i < m; ++i) Z[(lo + k) + (size_t)(lo - ob + i) * ldz] = zz[k][i];
   }
-  if (fail && k == 0) atomicMax(info, lo + 1);
+  if (fail && k == 0) atomicMax(info + pr, lo - ob + 1);
 }
 
 // --------------------------------------------------------------------------
@@ -195,6 +196,7 @@ struct MergeBufs {
   int* posD;     // final column for each deflated value
   double* Qtmp;
   double* U;
+  int lq;        // the merge of rows [lo, hi) keeps its Qtmp and U blocks at lo * lq
 };
 
 __device__ __forceinline__ int count_less(const double* a, int len, double v) {
@@ -214,11 +216,11 @@ __device__ __forceinline__ int count_leq(const double* a, int len, double v) {
   return lo;
 }
 
-__global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict__ descs, int n,
+__global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict__ descs,
                                                     const double* __restrict__ lam, const double* __restrict__ es,
                                                     const double* __restrict__ Z, int ldz, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.x];
-  const int lo = md.lo, mid = md.mid, hi = md.hi;
+  const int lo = md.lo, mid = md.mid, hi = md.hi, ob = md.ob;
   const int ns = hi - lo, n1 = mid - lo, n2 = hi - mid;
   const double beta = es[mid - 1];
   const double rho = 2.0 * beta;
@@ -231,11 +233,11 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
     if (t < n1) {
       v = lam[lo + t];
       r = t + count_less(lam + mid, n2, v);
-      zr = Z[(mid - 1) + (size_t)(lo + t) * ldz];
+      zr = Z[(mid - 1) + (size_t)(lo - ob + t) * ldz];
     } else {
       v = lam[mid + t - n1];
       r = (t - n1) + count_leq(lam + lo, n1, v);
-      zr = Z[mid + (size_t)(lo + t) * ldz];
+      zr = Z[mid + (size_t)(lo - ob + t) * ldz];
     }
     dk[r] = v;
     zk[r] = zr * 0.70710678118654752440;
@@ -319,7 +321,7 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
 __global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restrict__ descs, double* __restrict__ Z,
                                                       int ldz, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.y];
-  const int lo = md.lo, ns = md.hi - md.lo;
+  const int lo = md.lo, ns = md.hi - md.lo, cb = md.lo - md.ob;
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= ns) return;
   const int nr = B.nrot[blockIdx.y];
@@ -330,15 +332,15 @@ __global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restric
   for (int t = 0; t < nr; ++t) {
     const int pj = (int)rot[4 * t], j = (int)rot[4 * t + 1];
     const double c = rot[4 * t + 2], s = rot[4 * t + 3];
-    const size_t cp = (size_t)(lo + src[pj]) * ldz, cj = (size_t)(lo + src[j]) * ldz;
+    const size_t cp = (size_t)(cb + src[pj]) * ldz, cj = (size_t)(cb + src[j]) * ldz;
     const double x = zr[cp], y = zr[cj];
     zr[cp] = c * x + s * y;
     zr[cj] = c * y - s * x;
   }
-  double* Q = B.Qtmp + (size_t)lo * ldz;
+  double* Q = B.Qtmp + (size_t)lo * B.lq;
   for (int k = 0; k < ns; ++k) {
     const int sp = (k < K) ? B.keep[lo + k] : B.dfl[lo + k - K];
-    Q[r + (size_t)k * ns] = zr[(size_t)(lo + src[sp]) * ldz];
+    Q[r + (size_t)k * ns] = zr[(size_t)(cb + src[sp]) * ldz];
   }
 }
 
@@ -460,7 +462,7 @@ __global__ void __launch_bounds__(256) k_merge_zhat(const MergeDesc* __restrict_
   if (lane == 0) B.zhat[lo + j] = copysign(sqrt(fmax(prod, 0.0)), B.zkk[lo + j]);
 }
 
-__global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restrict__ descs, int ldz, MergeBufs B) {
+__global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restrict__ descs, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.y];
   const int lo = md.lo, ns = md.hi - md.lo;
   const int K = B.kcnt[blockIdx.y];
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restric
   const int o = B.org[lo + i];
   const double t = B.tau[lo + i];
   const double dorg = d[o];
-  double* Ucol = B.U + (size_t)lo * ldz + (size_t)i * ns;
+  double* Ucol = B.U + (size_t)lo * B.lq + (size_t)i * ns;
   double ss = 0.0;
   for (int j = threadIdx.x; j < K; j += 256) {
     const double v = zh[j] / ((d[j] - dorg) - t);
@@ -531,13 +533,15 @@ __global__ void __launch_bounds__(256) k_merge_copydefl(const MergeDesc* __restr
   if (t >= ns - K) return;
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= ns) return;
-  const double* Q = B.Qtmp + (size_t)lo * ldz;
-  Z[(lo + r) + (size_t)(lo + B.posD[lo + t]) * ldz] = Q[r + (size_t)(K + t) * ns];
+  const double* Q = B.Qtmp + (size_t)lo * B.lq;
+  Z[(lo + r) + (size_t)(lo - md.ob + B.posD[lo + t]) * ldz] = Q[r + (size_t)(K + t) * ns];
 }
 
-__global__ void k_dc_unscale(int n, double* __restrict__ lam, const double* __restrict__ scal, double* __restrict__ w) {
+__global__ void k_dc_unscale(const int* __restrict__ probs, const double* __restrict__ lam,
+                             const double* __restrict__ scal, double* __restrict__ w) {
+  const int o = probs[2 * blockIdx.y], n = probs[2 * blockIdx.y + 1];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) w[i] = lam[i] * scal[0];
+  if (i < n) w[o + i] = lam[o + i] * scal[blockIdx.y];
 }
 
 // --------------------------------------------------------------------------
@@ -557,11 +561,13 @@ __device__ __forceinline__ int sturm_count(int n, const double* d, const double*
   return cnt;
 }
 
-__global__ void __launch_bounds__(256) k_bisect(int n, const double* __restrict__ d, const double* __restrict__ e2,
-                                                double gl, double gu, double pivmin, const double* __restrict__ scal,
-                                                double* __restrict__ w) {
+__global__ void __launch_bounds__(256) k_bisect(const int* __restrict__ probs, const double* __restrict__ d,
+                                                const double* __restrict__ e2, double gl, double gu, double pivmin,
+                                                const double* __restrict__ scal, double* __restrict__ w) {
+  const int o = probs[2 * blockIdx.y], n = probs[2 * blockIdx.y + 1];
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
+  d += o; e2 += o; w += o;
   double a = gl, b = gu;   // count(a) <= k < count(b)
   for (int it = 0; it < 200; ++it) {
     const double m = 0.5 * (a + b);
@@ -569,7 +575,7 @@ __global__ void __launch_bounds__(256) k_bisect(int n, const double* __restrict_
     if (b - a <= 2.0 * DEPS * fmax(fabs(a), fabs(b))) break;
     if (sturm_count(n, d, e2, m, pivmin) > k) b = m; else a = m;
   }
-  w[k] = 0.5 * (a + b) * scal[0];
+  w[k] = 0.5 * (a + b) * scal[blockIdx.y];
 }
 
 __global__ void k_square(int n, const double* __restrict__ es, double* __restrict__ e2) {
@@ -578,70 +584,84 @@ __global__ void k_square(int n, const double* __restrict__ es, double* __restric
 }
 
 // --------------------------------------------------------------------------
-static void build_tree(int lo, int hi, int depth, std::vector<int>& leaves, std::vector<std::vector<MergeDesc>>& lv,
-                       std::vector<int>& splits) {
+static void build_tree(int lo, int hi, int ob, int pr, int depth, std::vector<int>& leaves,
+                       std::vector<std::vector<MergeDesc>>& lv, std::vector<int>& splits) {
   if (hi - lo <= LEAF) {
-    leaves.push_back(lo);
-    leaves.push_back(hi);
+    leaves.insert(leaves.end(), {lo, hi, ob, pr});
     return;
   }
   const int mid = lo + (hi - lo) / 2;
   if ((int)lv.size() <= depth) lv.resize(depth + 1);
-  lv[depth].push_back(MergeDesc{lo, mid, hi});
+  lv[depth].push_back(MergeDesc{lo, mid, hi, ob});
   splits.push_back(mid);
-  build_tree(lo, mid, depth + 1, leaves, lv, splits);
-  build_tree(mid, hi, depth + 1, leaves, lv, splits);
+  build_tree(lo, mid, ob, pr, depth + 1, leaves, lv, splits);
+  build_tree(mid, hi, ob, pr, depth + 1, leaves, lv, splits);
 }
 
-size_t stedc_int_slots(int n) {
-  // leaves (2 per leaf) + splits + merge descriptors (3 ints each)
-  const int nleaf = cdiv(n, LEAF) * 2 + 2;
-  return (size_t)2 * nleaf + nleaf + 3 * (size_t)nleaf + 64;
+size_t stedc_int_slots(int ntot, int nprob) {
+  // per problem of size n at most 2 ceil(n / 64) leaves (4 ints), one split
+  // and one merge (4 ints) fewer, and (offset, size)
+  return (size_t)ntot / 3 + 24 * (size_t)nprob + 64;
 }
 
-int stedc(int n, const double* d, const double* e, double* w, const StedcWork& sw, bool vectors, hipStream_t s) {
-  if (n <= 0) return ZQI_OK;
-  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(1024), 0, s, n, d, e, sw.lam, sw.es, sw.scal);
-  if (!vectors) {
-    // Gershgorin interval of the scaled matrix
-    hipLaunchKernelGGL(k_square, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.es, sw.e2);
-    const double pivmin = DBL_MIN * 4.0;
-    hipLaunchKernelGGL(k_bisect, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.lam, sw.e2, -3.0 - 1e-12, 3.0 + 1e-12,
-                       pivmin, sw.scal, w);
-    return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const double* d, const double* e, double* w,
+          const StedcWork& sw, bool vectors, hipStream_t s) {
+  if (nprob <= 0 || ntot <= 0) return ZQI_OK;
+  int nmax = 0;
+  std::vector<int> probs(2 * (size_t)nprob);
+  for (int z = 0; z < nprob; ++z) {
+    probs[2 * z] = offs[z];
+    probs[2 * z + 1] = ns[z];
+    nmax = std::max(nmax, ns[z]);
   }
+  if (nmax <= 0) return ZQI_OK;
   std::vector<int> leaves, splits;
   std::vector<std::vector<MergeDesc>> levels;
-  build_tree(0, n, 0, leaves, levels, splits);
-  const int nleaf = (int)leaves.size() / 2;
-  // descriptors go to the device through the integer scratch
-  int* ileaf = sw.ints;
-  int* isplit = ileaf + leaves.size();
-  MergeDesc* idesc = reinterpret_cast<MergeDesc*>(isplit + splits.size() + (splits.size() & 1));
-  size_t ndesc = 0;
-  for (auto& L : levels) ndesc += L.size();
+  if (vectors)
+    for (int z = 0; z < nprob; ++z)
+      if (ns[z] > 0) build_tree(offs[z], offs[z] + ns[z], offs[z], z, 0, leaves, levels, splits);
+  // device table: merge descriptors (deepest level first), leaves, splits, problems
   std::vector<MergeDesc> flat;
-  flat.reserve(ndesc);
   std::vector<size_t> level_off;
   for (int dpt = (int)levels.size() - 1; dpt >= 0; --dpt) {
     level_off.push_back(flat.size());
     for (auto& md : levels[dpt]) flat.push_back(md);
   }
-  std::vector<int> host(leaves.size() + splits.size() + (splits.size() & 1) + 3 * flat.size());
+  std::vector<int> host(4 * flat.size() + leaves.size() + splits.size() + probs.size());
   size_t o = 0;
+  for (auto& md : flat) { host[o++] = md.lo; host[o++] = md.mid; host[o++] = md.hi; host[o++] = md.ob; }
+  const size_t o_leaf = o;
   for (int v : leaves) host[o++] = v;
+  const size_t o_split = o;
   for (int v : splits) host[o++] = v;
-  if (splits.size() & 1) host[o++] = 0;
-  for (auto& md : flat) { host[o++] = md.lo; host[o++] = md.mid; host[o++] = md.hi; }
+  const size_t o_prob = o;
+  for (int v : probs) host[o++] = v;
   if (host.size() > sw.int_slots) return ZQI_ERR_ARG;
-  ZQ_HIP_CHECK(hipMemcpyAsync(sw.ints, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  ZQ_HIP_CHECK(hipMemsetAsync(sw.Z, 0, (size_t)n * n * sizeof(double), s));
+  int rc = stage_upload(*sw.stage, host.data(), host.size() * sizeof(int), sw.ints, s);
+  if (rc) return rc;
+  const MergeDesc* idesc = reinterpret_cast<const MergeDesc*>(sw.ints);
+  const int* ileaf = sw.ints + o_leaf;
+  const int* isplit = sw.ints + o_split;
+  const int* iprob = sw.ints + o_prob;
+
+  hipLaunchKernelGGL(k_dc_scale, dim3(nprob), dim3(1024), 0, s, iprob, d, e, sw.lam, sw.es, sw.scal);
+  if (!vectors) {
+    // Gershgorin interval of every scaled matrix lies in [-3, 3]
+    hipLaunchKernelGGL(k_square, dim3(cdiv(ntot, 256)), dim3(256), 0, s, ntot, sw.es, sw.e2);
+    const double pivmin = DBL_MIN * 4.0;
+    hipLaunchKernelGGL(k_bisect, dim3(cdiv(nmax, 256), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
+                       -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
+    return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+  }
+  const int nleaf = (int)leaves.size() / 4;
+  const int ldz = ntot;
+  ZQ_HIP_CHECK(hipMemsetAsync(sw.Z, 0, (size_t)ntot * lq * sizeof(double), s));
   if (!splits.empty())
     hipLaunchKernelGGL(k_dc_tear, dim3(cdiv((int)splits.size(), 256)), dim3(256), 0, s, (int)splits.size(), isplit,
                        sw.es, sw.lam);
-  hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, n, ileaf, sw.lam, sw.es, sw.Z, n, sw.info);
+  hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, ileaf, sw.lam, sw.es, sw.Z, ldz, sw.info);
   MergeBufs B{sw.dk, sw.zk, sw.src, sw.keep, sw.dfl, sw.rot, sw.nrot, sw.kcnt, sw.dkk, sw.zkk, sw.dval, sw.rho,
-              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U};
+              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U, lq};
   for (size_t li = 0; li < level_off.size(); ++li) {
     const size_t off = level_off[li];
     const size_t cnt = (li + 1 < level_off.size() ? level_off[li + 1] : flat.size()) - off;
@@ -649,23 +669,20 @@ int stedc(int n, const double* d, const double* e, double* w, const StedcWork& s
     int nsmax = 0;
     for (size_t q = 0; q < cnt; ++q) nsmax = std::max(nsmax, flat[off + q].hi - flat[off + q].lo);
     const int nm = (int)cnt;
-    hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, n, sw.lam, sw.es, sw.Z, n, B);
-    hipLaunchKernelGGL(k_merge_rotgat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, n, B);
+    hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, sw.lam, sw.es, sw.Z, ldz, B);
+    hipLaunchKernelGGL(k_merge_rotgat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
     hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
-    hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, n, B);
+    hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_order, dim3(nm), dim3(256), 0, s, dd, sw.lam, B);
-    for (int q = 0; q < nm; ++q) {
-      const MergeDesc& md = flat[off + q];
-      const int ns = md.hi - md.lo;
+    {
       ProfScope ps(PROF_MERGE, s, 0.0);
-      int rc = dgemm_nn(ns, 0, sw.kcnt + q, ns, sw.Qtmp + (size_t)md.lo * n, ns, sw.U + (size_t)md.lo * n, ns,
-                        sw.Z + md.lo + (size_t)md.lo * n, n, sw.posK + md.lo, s);
-      if (rc) return rc;
+      rc = dgemm_merges(dd, nm, nsmax, sw.kcnt, sw.Qtmp, sw.U, lq, sw.Z, ldz, sw.posK, s);
     }
-    hipLaunchKernelGGL(k_merge_copydefl, dim3(cdiv(nsmax, 256), nsmax, nm), dim3(256), 0, s, dd, sw.Z, n, B);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_merge_copydefl, dim3(cdiv(nsmax, 256), nsmax, nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
   }
-  hipLaunchKernelGGL(k_dc_unscale, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sw.lam, sw.scal, w);
+  hipLaunchKernelGGL(k_dc_unscale, dim3(cdiv(nmax, 256), nprob), dim3(256), 0, s, iprob, sw.lam, sw.scal, w);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

@@ -162,6 +162,17 @@ __device__ __forceinline__ double butterfly_sum(double* v) {
   if (N == 1) r = bf32(r, r);
   return r;
 }
+// Halving butterfly inside each 16-lane row for N (power of two, >= 16)
+// values per lane: afterwards v[t], t < N/16, of lane l holds the row sum of
+// value (l & 15) * (N/16) + t.
+template <int N>
+__device__ __forceinline__ void row16_halving(double* v) {
+  const int lane = threadIdx.x & 63;
+  bstep<8, N / 2>(v, lane);
+  bstep<4, N / 4>(v, lane);
+  bstep<2, N / 8>(v, lane);
+  bstep<1, N / 16>(v, lane);
+}
 // sum over all 64 lanes, result in every lane
 __device__ __forceinline__ double wave_sum_d(double v) {
   v += xlane<1>(v);
@@ -219,6 +230,21 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_
   hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, e0, e1, 0u, args...);
 }
 
+struct Staging;
+
+// Batch addressing of the per-column and per-problem kernels: problem
+// z = blockIdx.z has half-dimension nvec[z] and its buffers at byte offset
+// z * zs from the pointers passed (problem 0's).  The single solve passes
+// {nullptr, 0}, which leaves every pointer and size as given.
+struct ZB {
+  const int* nvec;
+  size_t zs;
+};
+template <class T>
+__device__ __forceinline__ T* zoff(T* p, const ZB& zb) {
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zb.zs);
+}
+
 // reduce.hip: quaternion Householder tridiagonalisation of the lower triangle
 // of M (n x n, ld).  Writes d (n), e (n-1 quaternions: the subdiagonal),
 // Tf (per panel nb x nb compact-WY factors) and leaves u_c in M[c+1:, c].
@@ -227,7 +253,9 @@ struct ReduceWork {
   quat* ucur;     // n
   quat* Prow;     // matvec row partials: ceil(n/32) x n
   quat* Pcol;     // matvec column partials: ceil(n/64) x n
-  quat* TS;       // tall-skinny accumulators [W^* u | V^* u]: 2 x nb
+  quat* TS;       // tall-skinny products [W^* u | V^* u]: 2 x nb
+  double* TSP;    // their per-block partials: ceil(n/16) x ldtsp
+  int ldtsp;      // 8 nb
   double* KP;     // ceil(n/16)
   double* part;   // ceil(n/16)
   double* scal;   // 16
@@ -235,8 +263,25 @@ struct ReduceWork {
   quat* Gm;       // same shape: strict upper part of V^* V per panel
 };
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s);
+struct QGemmTask;
+// Lockstep reduction of a batch of nprob problems sorted by n descending (nh:
+// host copy of the sizes, zb.nvec the device copy); every buffer of problem z
+// lies zb.zs bytes after problem z-1's (the pointers given are problem 0's,
+// ld = nh[0]).  tasks: device room for the trailing-update task table
+// (reduce_batched_tasks entries).  Each problem's result equals its solo reduce().
+size_t reduce_batched_tasks(int nprob, const int* nh, int nb);
+int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, double* d, quat* e, const ReduceWork& w,
+                   QGemmTask* tasks, Staging& st, hipStream_t s);
 
-// stedc.hip
+// stedc.hip.  Several independent tridiagonals (a forest) are solved
+// together: problem z owns the global index range [o_z, o_z + n_z) of every
+// 1-D array, and its eigenvector block Z[o_z:o_z+n_z, 0:n_z] (ld = total size).
+// A merge of global rows [lo, hi) with split mid belongs to the problem with
+// base ob.
+struct MergeDesc {
+  int lo, mid, hi, ob;
+};
+struct Staging;
 struct StedcWork {
   double* Z;      // n x n eigenvectors (ld n)
   double* Qtmp;   // n x n
@@ -246,12 +291,33 @@ struct StedcWork {
   double* scal;   // 16
   int *src, *keep, *dfl, *org, *posK, *posD;                             // n each
   int *nrot, *kcnt;                                                      // n each
-  int* info;      // 1
+  int* info;      // one per problem: leaf QL failure (first failing local row + 1)
   int* ints;      // tree descriptors
   size_t int_slots;
+  Staging* stage; // host staging of the descriptors
 };
-size_t stedc_int_slots(int n);
-int stedc(int n, const double* d, const double* e, double* w, const StedcWork& sw, bool vectors, hipStream_t s);
+size_t stedc_int_slots(int ntot, int nprob);
+// offs/ns (host): problem offsets and sizes; ntot = sum of ns; lq >= max ns.
+// d, e: concatenated diagonals and |subdiagonals| (e[o_z + i], i < n_z - 1);
+// w (ntot): eigenvalues ascending per problem.  Z (when vectors) is ld ntot.
+int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const double* d, const double* e, double* w,
+          const StedcWork& sw, bool vectors, hipStream_t s);
+
+// Pinned host staging for host -> device descriptor tables.  Two buffers
+// alternate between solves; stage_begin waits until the solve that last used
+// the buffer (two solves back) has completed on its stream, so the host can
+// run one solve ahead without blocking.
+struct Staging {
+  char* host[2] = {nullptr, nullptr};
+  size_t cap[2] = {0, 0};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int slot = 0;
+  size_t used = 0;
+};
+int stage_begin(Staging& st);
+int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStream_t s);
+int stage_end(Staging& st, hipStream_t s);
+void stage_free(Staging& st);
 
 // backtr.hip
 struct BacktrWork {
@@ -264,8 +330,38 @@ struct BacktrWork {
 };
 int backtransform(int n, int nb, const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz,
                   quat* X, int ldx, const BacktrWork& w, hipStream_t s);
+// Batched form (layout as reduce_batched): problem z's real eigenvectors are
+// Z + zrow_h[z] (ld ldz, the forest layout); zrow_d is the device copy.
+// tasks / sums: device room for backtransform_batched_tasks entries each.
+struct SumTask {
+  const quat* parts;
+  quat* out;
+  int total;
+  int nz;
+};
+size_t backtransform_batched_tasks(int nprob, const int* nh, int nb, int nsplit);
+int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_h, const int* zrow_d, int nb,
+                          const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz, quat* X,
+                          int ldx, const BacktrWork& w, QGemmTask* tasks, SumTask* sums, Staging& st, hipStream_t s);
 
 // layout.hip
+// One problem of a batch: the caller's D (in and out, ld2 complex), its
+// eigenvalue output, half-dimension n, and its offset in the concatenated
+// tridiagonal (forest) arrays.
+struct BatchProb {
+  double* D;
+  double* eig;
+  int ld2;
+  int n;
+  int off;
+  int pad;
+};
+int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s);
+int store_vectors_batched(int nprob, int L, const BatchProb* P, const quat* X, int ldx, ZB zb, hipStream_t s);
+// dall[off + i] = d_z[i], eall[off + i] = |e_z[i]|
+int gather_tridiag_batched(int nprob, int L, const BatchProb* P, const double* d, const quat* e, ZB zb, double* dall,
+                           double* eall, hipStream_t s);
+int scatter_eig_batched(int nprob, int L, const BatchProb* P, const double* wall, hipStream_t s);
 int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s);
 int store_vectors(int n, const quat* X, int ldx, double* D, int ld2, hipStream_t s);
 int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
@@ -284,14 +380,27 @@ struct QOperand {
 };
 int qgemm(int m, int nn, int k, double alpha, const QOperand& A, const QOperand& B, double beta, quat* C, int ldc,
           bool lower_only, hipStream_t s);
+// One quaternion GEMM piece: C = beta C + alpha opA(A) opB(B) summed over
+// logical k in [kbeg, kend); K bounds the operand reads; lower_only writes
+// only i >= j.
+struct QGemmTask {
+  int m, nn, K, kbeg, kend;
+  int ldc, lower_only;
+  double alpha, beta;
+  QOperand A, B;
+  quat* C;
+};
+// ntasks independent tasks (device array) in one launch; maxm / maxnn bound
+// their shapes.  Each task computes exactly what qgemm would for it.
+int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, hipStream_t s);
 // number of K chunks qgemm_cn_splitk uses for a given request
 int splitk_count(int K, int nsplit, int* kchunk_out);
 // Ypart[z] (m x nn, ld m) = A B over the z-th K chunk (A logical m x K, B K x nn)
 int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int ldb, quat* Ypart, int nsplit,
                     hipStream_t s);
-// real: C[:, cmap[j]] = A (m x K) B (K x K) with K read on the device (<= k_max)
-int dgemm_nn(int m, int unused, const int* k_dev, int k_max, const double* A, int lda, const double* B, int ldb,
-             double* C, int ldc, const int* cmap, hipStream_t s);
+// real merge products of one divide & conquer tree level (see stedc.hip)
+int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const double* Qtmp,
+                 const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s);
 int mfma_probe(const double* a, const double* b, double* out, hipStream_t s);
 
 }  // namespace zq
