@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--workload", choices=("single", "batched"), default="single",
+                    help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched summary of the single workload")
     ap.add_argument("--profile-run", action="store_true",
                     help="only warmup + timed steps (for rocprofv3), no instrumentation pass or baseline")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -93,6 +97,86 @@ def cpu_baseline(n2):
                        f"{r['threads']} MKL threads, {r['seconds']:.2f} s; GFLOP/s on (160/3) n^3")}
 
 
+def batched_workload(count, seed, dev):
+    """The batched workload: `count` Fock-like problems, 2n uniform over even
+    [64, 1024] (sizes from seed, matrix i from seed + i), built on the device
+    in one buffer.  Returns (n2 list, buffer, per-problem column views, pristine copy)."""
+    import torch
+    from zquatev_amd import gen
+    n2s = [int(v) for v in gen.batched_sizes(count, seed)]
+    offs, o = [], 0
+    for v in n2s:
+        offs.append(o)
+        o += v * v
+    buf = torch.zeros(o, dtype=torch.complex128, device=dev)
+    views = []
+    for k, v in enumerate(n2s):
+        Dk = buf[offs[k]: offs[k] + v * v].view(v, v)   # Dk[j, i] = D[i, j], ld2 = 2n
+        gen.fock_case_device(v // 2, seed + k, Dk)
+        views.append(Dk)
+    return n2s, buf, views, buf.clone()
+
+
+def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
+    """Batched solves/s of the batched workload against the plain loop of
+    single device solves on the same stream."""
+    import torch
+    from zquatev_amd import gen
+    dev = torch.device("cuda:0")
+    n2s, buf, views, pristine = batched_workload(count, seed, dev)
+    eigs = [torch.zeros(v // 2, dtype=torch.float64, device=dev) for v in n2s]
+    dptr = [t.data_ptr() for t in views]
+    eptr = [e.data_ptr() for e in eigs]
+    flops = sum(gen.flops_vectors(v // 2) for v in n2s)
+
+    def batch_step():
+        buf.copy_(pristine)
+        rc = ctx.zquatev_batched_device(n2s, dptr, n2s, eptr, "V")
+        if rc != 0:
+            raise RuntimeError(f"zq_zquatev_batched_device failed ({rc})")
+
+    def loop_step():
+        buf.copy_(pristine)
+        for v, d, e in zip(n2s, dptr, eptr):
+            rc = ctx.zquatev_device(v, d, v, e, "V")
+            if rc != 0:
+                raise RuntimeError(f"zq_zquatev_device failed ({rc})")
+
+    def timed(fn, k, w):
+        for _ in range(w):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    t_loop = timed(loop_step, 1, 1)
+    res_loop = [e.clone() for e in eigs[:8]]
+    t_batch = timed(batch_step, steps, warmup)
+    out = {"problems": count, "n2_range": [min(n2s), max(n2s)], "seed": seed,
+           "sum_n3": float(sum((v // 2) ** 3 for v in n2s)),
+           "batched_solves_per_s": round(count / t_batch, 1), "batched_ms": round(1000 * t_batch, 2),
+           "batched_gflops": round(flops / t_batch / 1e9, 1),
+           "loop_solves_per_s": round(count / t_loop, 1), "loop_ms": round(1000 * t_loop, 2),
+           "speedup_vs_loop": round(t_loop / t_batch, 2)}
+    if check:
+        # batch members equal the loop's solves (bitwise), and two members against H
+        same = all(torch.equal(a, b) for a, b in zip(res_loop, eigs[:8]))
+        worst = 0.0
+        for k in (0, count // 2):
+            n = n2s[k] // 2
+            H = gen.full_hermitian_t(pristine[sum(v * v for v in n2s[:k]):][: n2s[k] ** 2].view(n2s[k], n2s[k]), n)
+            X = views[k].T
+            lam = torch.cat([eigs[k], eigs[k]]).to(H.dtype)
+            hn = torch.linalg.matrix_norm(H, 2).item()
+            worst = max(worst, (torch.linalg.norm(H @ X - X * lam[None, :]) / (n * hn)).item())
+        out["check"] = {"loop_equal_first8": bool(same), "residual_max_2": worst,
+                        "ok": bool(same and worst <= 1e-13)}
+    return out
+
+
 def measured_traffic(kernel, n2, nb):
     """HBM bytes per launch of `kernel` at this workload from the committed
     counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
@@ -122,6 +206,22 @@ def main():
     import torch
     import zquatev_amd as zq
     from zquatev_amd import gen
+
+    if args.workload == "batched":
+        ctx = zq.Context(0)
+        stream = torch.cuda.current_stream(torch.device("cuda:0"))
+        ctx.set_stream(stream.cuda_stream)
+        r = run_batched(ctx, stream, args.batch_count, 1002, args.steps, args.warmup, check=not args.no_check)
+        line = {"metric": "batched solves/s (batched workload)", "value": r["batched_solves_per_s"],
+                "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": r["batched_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64", "data": "synthetic",
+                "config": {"workload": f"{args.batch_count} Fock-like problems, 2n uniform over even [64, 1024], "
+                                       "eigenvalues + eigenvectors, one batched call",
+                           "seed": 1002, "generator": "splitmix64 (device)"},
+                "batched": r}
+        print(json.dumps(line))
+        return 0
 
     n2 = args.n2
     n = n2 // 2
@@ -231,6 +331,12 @@ def main():
             else:
                 v["TFLOP_per_s"] = round(v["units"] / (v["total_ms"] / 1000.0) / 1e12, 2)
 
+    batched = None
+    if not args.no_batched:
+        del Dt, pristine
+        torch.cuda.empty_cache()
+        batched = run_batched(ctx, stream, args.batch_count, 1002, 2, 1, check=not args.no_check)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n2)
@@ -258,6 +364,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "check": check,
+        "batched": batched,
     }
     print(json.dumps(line))
     return 0

@@ -143,3 +143,85 @@ def flops_vectors(n):
 def flops_values(n):
     """Algorithmic flop count of one eigenvalues-only solve."""
     return 64.0 / 3.0 * float(n) ** 3
+
+
+# ---------------------------------------------------------------------------
+# Device-side generation (torch) of the same distributions, for workloads too
+# large to build on the host (batched and serving streams).  Same stream
+# positions as the numpy versions; the float transforms (log1p, cos) come from
+# the device math library, so values agree with the host ones to rounding.
+# Only what the solver reads is written: the lower triangles (with diagonal)
+# of A and B in the left half of a column-major 2n x 2n matrix.
+# ---------------------------------------------------------------------------
+def _i64(u):
+    return u - (1 << 64) if u >= (1 << 63) else u
+
+
+_T_GOLDEN = _i64(0x9E3779B97F4A7C15)
+_T_M1 = _i64(0xBF58476D1CE4E5B9)
+_T_M2 = _i64(0x94D049BB133111EB)
+
+
+def _lsr(z, k):
+    return (z >> k) & ((1 << (64 - k)) - 1)
+
+
+def uniform_t(seed, idx):
+    """torch version of :func:`uniform` on an int64 index tensor (wrapping arithmetic)."""
+    z = idx.new_full((), _i64(seed % (1 << 64))) + (idx + 1) * _T_GOLDEN
+    z = (z ^ _lsr(z, 30)) * _T_M1
+    z = (z ^ _lsr(z, 27)) * _T_M2
+    z = z ^ _lsr(z, 31)
+    return _lsr(z, 11).to(dtype=__import__("torch").float64) * (1.0 / 9007199254740992.0)
+
+
+def fock_case_device(n, seed, out, decay=40.0, b_sigma=0.1, chunk=256):
+    """:func:`fock_case` written into ``out``: a torch complex128 tensor of
+    shape (>= n columns, 2n + pad rows) holding column j of D in out[j] (i.e.
+    the row-major view of a column-major D with ld2 = out.shape[1]).  Only
+    the lower triangles of A and B are written."""
+    import torch
+    dev = out.device
+    diag = torch.linspace(-20.0, 5.0, n, dtype=torch.float64, device=dev)
+    i = torch.arange(n, dtype=torch.int64, device=dev)[None, :]
+    for j0 in range(0, n, chunk):
+        j1 = min(n, j0 + chunk)
+        j = torch.arange(j0, j1, dtype=torch.int64, device=dev)[:, None]
+        s = 4 * (j * n + i)                       # (w, n): column j, row i
+        low = i > j
+        env = torch.exp(-(i - j).abs().to(torch.float64) / decay)
+
+        def normal(k0, k1):
+            u1 = uniform_t(seed, k0)
+            u2 = uniform_t(seed, k1)
+            return torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * torch.pi * u2)
+
+        g = normal(s, s + 1)
+        h = normal(s + 2, s + 3)
+        a = torch.where(low, g * env / (2.0 ** 0.5), torch.zeros((), dtype=torch.float64, device=dev))
+        a = torch.complex(a, a)
+        a = torch.where(i == j, torch.complex(diag[None, :].expand_as(g), torch.zeros_like(g)), a)
+        b = torch.where(low, b_sigma * h * env, torch.zeros((), dtype=torch.float64, device=dev))
+        out[j0:j1, :n] = a
+        out[j0:j1, n:2 * n] = torch.complex(b, torch.zeros_like(b))
+    return out
+
+
+def full_hermitian_t(Dcols, n):
+    """H (2n x 2n, torch) from the lower triangles written by fock_case_device
+    (Dcols[j, i] = D[i, j])."""
+    import torch
+    A = Dcols[:n, :n].T
+    B = Dcols[:n, n:2 * n].T
+    L = torch.tril(A, -1)
+    A = L + L.conj().T + torch.diag(torch.diagonal(A).real.to(A.dtype))
+    Bl = torch.tril(B, -1)
+    B = Bl - Bl.T
+    return torch.cat([torch.cat([A, -B.conj()], 1), torch.cat([B, A.conj()], 1)], 0)
+
+
+def serving_sizes(count, seed, lo=128, hi=4096):
+    """Even 2n values log-uniform over [lo, hi] (serving workload)."""
+    u = uniform(seed, np.arange(count))
+    v = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
+    return (2 * np.clip(np.round(v / 2.0), lo // 2, hi // 2)).astype(np.int64)
