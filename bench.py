@@ -38,8 +38,16 @@ def parse():
     ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--workload", choices=("single", "batched"), default="single",
-                    help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only")
+    ap.add_argument("--workload", choices=("single", "batched", "serving"), default="single",
+                    help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only; "
+                         "serving: the Poisson request stream through the server and the plain loop")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--loop-seconds", type=float, default=10.0, help="arrival window replayed through the plain loop")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--pool", type=int, default=1024)
+    ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--max-batch-gb", type=float, default=24.0)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched summary of the single workload")
     ap.add_argument("--profile-run", action="store_true",
@@ -177,6 +185,149 @@ def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
     return out
 
 
+def run_serving(args):
+    """The serving workload: Poisson arrivals (rate args.rate) for
+    args.serve_seconds; 2n log-uniform over even [128, 4096]; 70 % of the
+    requests want eigenvectors.  Requests cycle through a pool of distinct
+    Fock-like matrices resident in HBM (restored from a pristine copy before
+    each use).  Served by the continuous-batching server, then by the plain
+    loop of one synchronous device solve per request (FIFO) on the same
+    arrival times."""
+    import numpy as np
+    import torch
+    import zquatev_amd as zq
+    from zquatev_amd import gen
+    dev = torch.device("cuda:0")
+    T, lam, P = args.serve_seconds, args.rate, args.pool
+    # arrivals and per-request choices (seed 1003)
+    nmax_req = int(T * lam * 1.5) + 64
+    gaps = -np.log1p(-gen.uniform(1003, np.arange(nmax_req))) / lam
+    arr = np.cumsum(gaps)
+    count = int(np.searchsorted(arr, T))
+    arr = arr[:count]
+    vec = gen.uniform(1003 + 7, np.arange(count)) < 0.7
+    P = min(P, count)
+    sizes = [int(v) for v in gen.serving_sizes(P, 1003)]
+    pristine, work, eigs = [], [], []
+    for s_, v in enumerate(sizes):
+        buf = torch.zeros(v, v, dtype=torch.complex128, device=dev)
+        gen.fock_case_device(v // 2, 2003 + s_, buf)
+        pristine.append(buf[: v // 2].clone())
+        work.append(buf)
+        eigs.append(torch.zeros(v // 2, dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    cur = torch.cuda.current_stream(dev)
+
+    def restore(s_):
+        work[s_][: sizes[s_] // 2].copy_(pristine[s_])
+        cur.synchronize()
+
+    # ---- continuous batching server
+    srv = zq.Server(0, lanes=args.lanes, max_batch=args.max_batch, max_batch_gb=args.max_batch_gb)
+    # warm the lanes' workspaces with one burst of the largest items
+    big = sorted(range(P), key=lambda q: -sizes[q])[: 2 * args.lanes]
+    for q in big:
+        restore(q)
+        srv.submit(sizes[q], work[q].data_ptr(), sizes[q], eigs[q].data_ptr(), "V")
+    srv.drain()
+    srv.poll()
+    last_ticket = {}
+    done_t = {}
+    recs = []
+    t_base = srv.clock_us() + 1000.0
+    tick_of = {}
+    for r in range(count):
+        s_ = r % P
+        due = t_base + arr[r] * 1e6
+        while srv.clock_us() < due:
+            for c in srv.poll():
+                done_t[c["ticket"]] = c
+            dt = (due - srv.clock_us()) * 1e-6
+            if dt > 0.0005:
+                time.sleep(dt - 0.0003)
+        if s_ in last_ticket:
+            while last_ticket[s_] not in done_t:
+                for c in srv.poll():
+                    done_t[c["ticket"]] = c
+        restore(s_)
+        tk = srv.submit(sizes[s_], work[s_].data_ptr(), sizes[s_], eigs[s_].data_ptr(), "V" if vec[r] else "N")
+        last_ticket[s_] = tk
+        tick_of[tk] = (r, due)
+    srv.drain()
+    for c in srv.poll():
+        done_t[c["ticket"]] = c
+    lat_arr, lat_done, bsz, infos = [], [], [], []
+    for tk, (r, due) in tick_of.items():
+        c = done_t[tk]
+        lat_arr.append(due * 1e-6)
+        lat_done.append(c["t_done_us"] * 1e-6)
+        bsz.append(c["batch_size"])
+        infos.append(c["info"])
+    serve = zq.latency_stats(lat_done, arrivals=lat_arr)
+    serve["mean_batch"] = float(np.mean(bsz))
+    serve["max_batch_seen"] = int(np.max(bsz))
+    serve["bad_info"] = int(sum(1 for v in infos if v != 0))
+    # accuracy: the last use of a few items (vectors) against H, and alone == in the mix
+    chk_ok, worst = True, 0.0
+    ctx = zq.Context(0)
+    ctx.set_stream(cur.cuda_stream)
+    checked = 0
+    for r in range(count - 1, -1, -1):
+        s_ = r % P
+        if r != max(q for q in range(s_, count, P)) or not vec[r] or checked >= 6:
+            continue
+        n = sizes[s_] // 2
+        X = work[s_].T.clone()
+        e_srv = eigs[s_].clone()
+        full = torch.zeros(sizes[s_], sizes[s_], dtype=torch.complex128, device=dev)
+        full[:n] = pristine[s_]
+        H = gen.full_hermitian_t(full, n)
+        lamv = torch.cat([e_srv, e_srv]).to(H.dtype)
+        hn = torch.linalg.matrix_norm(H, 2).item()
+        worst = max(worst, (torch.linalg.norm(H @ X - X * lamv[None, :]) / (n * hn)).item())
+        restore(s_)
+        rc = ctx.zquatev_device(sizes[s_], work[s_].data_ptr(), sizes[s_], eigs[s_].data_ptr(), "V")
+        torch.cuda.synchronize()
+        chk_ok = chk_ok and rc == 0 and torch.equal(eigs[s_], e_srv) and torch.equal(work[s_].T, X)
+        checked += 1
+    serve["check"] = {"residual_max": worst, "alone_equals_mix": bool(chk_ok), "checked": checked,
+                      "ok": bool(chk_ok and worst <= 1e-13)}
+    srv.close()
+
+    # ---- plain loop: FIFO, one synchronous device solve per request, same arrivals
+    loop_arr, loop_done = [], []
+    t0 = time.perf_counter() + 0.001
+    limit = min(count, int(args.loop_seconds * lam)) if args.loop_seconds > 0 else count
+    for r in range(limit):
+        s_ = r % P
+        due = t0 + arr[r]
+        while time.perf_counter() < due:
+            pass
+        restore(s_)
+        rc = ctx.zquatev_device(sizes[s_], work[s_].data_ptr(), sizes[s_], eigs[s_].data_ptr(),
+                                "V" if vec[r] else "N")
+        cur.synchronize()
+        if rc != 0:
+            raise RuntimeError(f"zq_zquatev_device failed ({rc})")
+        loop_arr.append(due)
+        loop_done.append(time.perf_counter())
+    loop = zq.latency_stats(loop_done, arrivals=loop_arr)
+    loop["requests_offered"] = limit
+    ctx.close()
+    line = {"metric": "serving solves/s + p50/p99 latency vs plain loop", "value": round(serve["solves_per_s"], 1),
+            "unit": "solves/s", "n_gpus": 1, "steps": count, "warmup": len(big),
+            "ms_per_step": round(1000.0 / serve["solves_per_s"], 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Poisson arrivals {lam}/s for {T} s, 2n log-uniform over even [128, 4096], "
+                                   "70% eigenvectors, Fock-like matrices",
+                       "requests": count, "pool": P, "seed": 1003, "lanes": args.lanes,
+                       "max_batch": args.max_batch},
+            "serving": serve, "plain_loop": loop,
+            "speedup_solves_per_s": round(serve["solves_per_s"] / loop["solves_per_s"], 2)}
+    print(json.dumps(line))
+    return 0
+
+
 def measured_traffic(kernel, n2, nb):
     """HBM bytes per launch of `kernel` at this workload from the committed
     counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
@@ -207,6 +358,8 @@ def main():
     import zquatev_amd as zq
     from zquatev_amd import gen
 
+    if args.workload == "serving":
+        return run_serving(args)
     if args.workload == "batched":
         ctx = zq.Context(0)
         stream = torch.cuda.current_stream(torch.device("cuda:0"))

@@ -91,6 +91,41 @@ int zq_zquatev_device(zq_context* ctx, int n2, double* dD, int nld2, double* dei
 int zq_zquatev_batched_device(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,
                               double* const* eig, int jobz, int* info);
 
+/* ---- Serving loop ----------------------------------------------------------
+ * No counterpart in the reference (one synchronous ts::zquatev call per
+ * matrix, zquatev.cc:42-100).  Requests are queued by zq_server_submit and
+ * served by continuous batching: a scheduler thread batches whatever has
+ * arrived (FIFO, same jobz, at most max_batch problems and max_batch_gb of
+ * workspace) onto one of `lanes` contexts (own stream, resident workspace);
+ * batches on different lanes run concurrently.  Each request's result is
+ * bitwise that of its own zq_zquatev_device call.  Device pointers stay
+ * owned by the caller and must stay valid until the request's completion
+ * record has been returned by zq_server_poll.  Times are microseconds of a
+ * steady clock started by zq_server_create. */
+typedef struct zq_server zq_server;
+typedef struct {
+  unsigned long long ticket;
+  int info;         /* 0; > 0 no convergence; < 0 error code of the batched call */
+  int batch_size;   /* problems in the batch that served the request */
+  int lane;
+  double t_submit_us, t_start_us, t_done_us;
+} zq_completion;
+
+/* lanes 1..8 (-3), max_batch >= 1 (-4), max_batch_gb >= 0, 0 = no limit (-5). */
+int zq_server_create(zq_server** out, int device, int lanes, int max_batch, double max_batch_gb);
+/* Queue one request; arguments as zq_zquatev_device (errors -2..-6 for n2,
+ * dD, ld2, deig, jobz).  Returns at once with its ticket. */
+int zq_server_submit(zq_server* srv, int n2, double* dD, int ld2, double* deig, int jobz,
+                     unsigned long long* ticket);
+/* Move up to max completion records into out; returns how many (never waits). */
+int zq_server_poll(zq_server* srv, zq_completion* out, int max);
+/* Wait until every submitted request has completed. */
+int zq_server_drain(zq_server* srv);
+/* The server's clock (microseconds since creation). */
+double zq_server_clock_us(zq_server* srv);
+/* Drain, stop the threads, free the lanes. */
+int zq_server_destroy(zq_server* srv);
+
 /* Host-memory batch with eigenvectors (ts::zquatev semantics per problem),
  * synchronous.  Same return codes; info optional. */
 int zq_zquatev_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,

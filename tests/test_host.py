@@ -90,3 +90,32 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_latency_stats_synthetic_clock():
+    # arrivals every 10 ms, each served 3 ms after arrival except the last (13 ms)
+    arr = np.arange(100) * 0.010
+    done = arr + 0.003
+    done[-1] = arr[-1] + 0.013
+    st = zq.latency_stats(done, arrivals=arr)
+    assert st["completed"] == 100
+    assert abs(st["p50_ms"] - 3.0) < 1e-9
+    assert abs(st["max_ms"] - 13.0) < 1e-9
+    assert 3.0 < st["p99_ms"] <= 13.0
+    assert abs(st["solves_per_s"] - 100 / (0.99 + 0.013)) < 1e-9
+    recs = [{"t_submit_us": a * 1e6, "t_done_us": d * 1e6} for a, d in zip(arr, done)]
+    st2 = zq.latency_stats(recs)
+    assert abs(st2["p50_ms"] - st["p50_ms"]) < 1e-6 and abs(st2["solves_per_s"] - st["solves_per_s"]) < 1e-6
+
+
+def test_server_validates_without_device():
+    L = zq.lib()
+    h = ctypes.c_void_p()
+    assert L.zq_server_create(None, 0, 1, 4, ctypes.c_double(0.0)) == zq.ERR_ARG
+    assert L.zq_server_create(ctypes.byref(h), 0, 0, 4, ctypes.c_double(0.0)) == -3
+    assert L.zq_server_create(ctypes.byref(h), 0, 2, 0, ctypes.c_double(0.0)) == -4
+    assert L.zq_server_create(ctypes.byref(h), 0, 2, 4, ctypes.c_double(-1.0)) == -5
+    t = ctypes.c_ulonglong()
+    assert L.zq_server_submit(None, 4, None, 4, None, ord("V"), ctypes.byref(t)) == zq.ERR_ARG
+    assert L.zq_server_poll(None, None, 0) == zq.ERR_ARG
+    assert L.zq_server_destroy(None) == 0

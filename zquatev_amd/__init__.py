@@ -35,6 +35,12 @@ _SIGNATURES = {
     "zq_zquatev_device": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int]),
     "zq_zquatev_batched_device": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "zq_zquatev_batched": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "zq_server_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, ctypes.c_double]),
+    "zq_server_submit": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "zq_server_poll": (_c_int, [_vp, _vp, _c_int]),
+    "zq_server_drain": (_c_int, [_vp]),
+    "zq_server_clock_us": (ctypes.c_double, [_vp]),
+    "zq_server_destroy": (_c_int, [_vp]),
     "zq_tridiagonalize": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
     "zq_tridiag_eig": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
     "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
@@ -252,6 +258,85 @@ class Context:
         k = _fn("zq_last_timings")(self._h, buf, 5)
         keys = ("reduce", "tridiag", "backtransform", "layout", "total")
         return {keys[i]: buf[i] for i in range(k)}
+
+
+class Completion(ctypes.Structure):
+    """zq_completion: one served request."""
+    _fields_ = [("ticket", ctypes.c_ulonglong), ("info", _c_int), ("batch_size", _c_int), ("lane", _c_int),
+                ("t_submit_us", ctypes.c_double), ("t_start_us", ctypes.c_double), ("t_done_us", ctypes.c_double)]
+
+
+class Server:
+    """Continuous-batching server (zq_server_*): submit device-pointer requests,
+    poll completion records."""
+
+    def __init__(self, device=0, lanes=2, max_batch=64, max_batch_gb=0.0):
+        h = _vp()
+        _check(_fn("zq_server_create")(ctypes.byref(h), device, lanes, max_batch, float(max_batch_gb)),
+               "zq_server_create")
+        self._h = h
+        self._buf = (Completion * 1024)()
+
+    def submit(self, n2, d_ptr, ld2, eig_ptr, jobz="V"):
+        t = ctypes.c_ulonglong()
+        _check(_fn("zq_server_submit")(self._h, int(n2), _vp(d_ptr), int(ld2), _vp(eig_ptr), ord(jobz),
+                                       ctypes.byref(t)), "zq_server_submit")
+        return t.value
+
+    def poll(self):
+        """Completed requests since the last poll, as dicts."""
+        out = []
+        while True:
+            k = _fn("zq_server_poll")(self._h, self._buf, len(self._buf))
+            if k < 0:
+                raise ZQError(k, "zq_server_poll")
+            for c in self._buf[:k]:
+                out.append({f: getattr(c, f) for f, _ in Completion._fields_})
+            if k < len(self._buf):
+                return out
+
+    def drain(self):
+        _check(_fn("zq_server_drain")(self._h), "zq_server_drain")
+
+    def clock_us(self):
+        return _fn("zq_server_clock_us")(self._h)
+
+    def close(self):
+        if self._h:
+            _fn("zq_server_destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def latency_stats(records, arrivals=None):
+    """Serving statistics from completion records (dicts with t_submit_us /
+    t_done_us, or (arrival, done) pairs in seconds when `arrivals` is given
+    as the arrival times): p50 / p99 / mean latency in ms and solves/s =
+    completed / (last completion - first arrival)."""
+    if arrivals is not None:
+        a = np.asarray(arrivals, dtype=np.float64)
+        d = np.asarray(records, dtype=np.float64)
+    else:
+        a = np.array([r["t_submit_us"] for r in records]) * 1e-6
+        d = np.array([r["t_done_us"] for r in records]) * 1e-6
+    if len(a) == 0:
+        return {"completed": 0}
+    lat = (d - a) * 1000.0
+    span = d.max() - a.min()
+    return {"completed": int(len(a)), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "max_ms": float(lat.max()), "solves_per_s": float(len(a) / span) if span > 0 else float("inf")}
 
 
 # torch helpers (torch is plumbing for device memory and streams only)

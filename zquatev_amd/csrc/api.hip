@@ -47,6 +47,8 @@ struct zq_context {
   int* info_dev = nullptr;  // leaf-solver status of the last vectors solve
   zq::ProfState prof;
   zq::Staging stage;
+  int* binfo = nullptr;     // per-problem status of the last batch (device)
+  size_t binfo_bytes = 0;
 };
 
 
@@ -375,7 +377,7 @@ size_t batch_layout(int nprob, const int* nh, int nb, bool vectors, char* base, 
 
 // One lockstep group on the context's workspace (already large enough).
 // P: the group's problems, sizes descending, P[z].off the forest offsets.
-int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vectors, int* info_out) {
+int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vectors) {
   hipStream_t s = ctx->stream;
   const int nprob = (int)P.size();
   std::vector<int> nh(nprob), zrow(nprob);
@@ -411,80 +413,81 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
                                         bp.btasks, bp.sums, st, s)))
       return rc;
     if ((rc = zq::store_vectors_batched(nprob, L, bp.probs, bp.slab.X, L, zb, s))) return rc;
-    if (info_out) {
-      ZQ_HIP_CHECK(hipMemcpyAsync(info_out, bp.sw.info, nprob * sizeof(int), hipMemcpyDeviceToHost, s));
-      ZQ_HIP_CHECK(hipStreamSynchronize(s));
-    }
-  } else if (info_out) {
-    for (int z = 0; z < nprob; ++z) info_out[z] = 0;
+    if ((rc = zq::scatter_info_batched(nprob, bp.probs, bp.sw.info, ctx->binfo, s))) return rc;
   }
   return ZQ_OK;
 }
 
-// Device-memory batch: validates, groups, solves.  info (host, optional)
-// receives each problem's status; with info the call waits for completion.
-int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
-                  bool vectors, int* info) {
-  std::vector<std::vector<zq::BatchProb>> groups;
-  std::vector<std::vector<int>> gidx;
-  std::vector<int> gnb;
+// Device-memory batch: groups and solves (arguments already validated).
+// Each problem's status is gathered on the device; info (host, optional)
+// receives it after the call waits for completion; info_async (pinned host,
+// optional) receives it asynchronously, valid once the stream has passed the
+// call.
+struct Groups {
+  std::vector<std::vector<zq::BatchProb>> probs;   // per group, sizes descending
+  std::vector<int> nb;
+  size_t need = 0;                                 // workspace bytes of the largest group
+};
+
+Groups make_groups(const zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,
+                   double* const* eig, bool vectors) {
+  Groups G;
   for (int i = 0; i < count; ++i) {
-    if (info) info[i] = 0;
     if (n2[i] == 0) continue;
     const int n = n2[i] / 2;
     const int nb = panel_width(ctx, n);
     size_t g = 0;
-    while (g < gnb.size() && gnb[g] != nb) ++g;
-    if (g == gnb.size()) {
-      gnb.push_back(nb);
-      groups.emplace_back();
-      gidx.emplace_back();
+    while (g < G.nb.size() && G.nb[g] != nb) ++g;
+    if (g == G.nb.size()) {
+      G.nb.push_back(nb);
+      G.probs.emplace_back();
     }
-    groups[g].push_back(zq::BatchProb{D[i], eig[i], ld2[i], n, 0, 0});
-    gidx[g].push_back(i);
+    G.probs[g].push_back(zq::BatchProb{D ? D[i] : nullptr, eig ? eig[i] : nullptr, ld2 ? ld2[i] : n2[i], n, 0, i});
   }
-  if (groups.empty()) return ZQ_OK;
-  // sizes descending, ties by position (stable)
-  for (size_t g = 0; g < groups.size(); ++g) {
-    std::vector<int> ord(groups[g].size());
-    for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int)k;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return groups[g][a].n > groups[g][b].n; });
-    std::vector<zq::BatchProb> P(ord.size());
-    std::vector<int> I(ord.size());
-    for (size_t k = 0; k < ord.size(); ++k) {
-      P[k] = groups[g][ord[k]];
-      I[k] = gidx[g][ord[k]];
-    }
-    groups[g].swap(P);
-    gidx[g].swap(I);
-  }
-  size_t need = 0;
-  for (size_t g = 0; g < groups.size(); ++g) {
-    std::vector<int> nh(groups[g].size());
-    for (size_t k = 0; k < nh.size(); ++k) nh[k] = groups[g][k].n;
+  for (size_t g = 0; g < G.probs.size(); ++g) {
+    // sizes descending, ties by position
+    std::stable_sort(G.probs[g].begin(), G.probs[g].end(),
+                     [](const zq::BatchProb& a, const zq::BatchProb& b) { return a.n > b.n; });
+    std::vector<int> nh(G.probs[g].size());
+    for (size_t k = 0; k < nh.size(); ++k) nh[k] = G.probs[g][k].n;
     BatchPlan bp;
-    need = std::max(need, batch_layout((int)nh.size(), nh.data(), gnb[g], vectors, nullptr, &bp));
+    G.need = std::max(G.need, batch_layout((int)nh.size(), nh.data(), G.nb[g], vectors, nullptr, &bp));
   }
+  return G;
+}
+
+int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                  bool vectors, int* info, int* info_async) {
+  Groups G = make_groups(ctx, count, n2, D, ld2, eig, vectors);
+  if (G.probs.empty()) {
+    if (info)
+      for (int i = 0; i < count; ++i) info[i] = 0;
+    if (info_async)
+      for (int i = 0; i < count; ++i) info_async[i] = 0;
+    return ZQ_OK;
+  }
+  const size_t need = G.need;
   int rc = ensure(ctx->ws, ctx->ws_bytes, need);
   if (rc) return rc;
+  if ((rc = ensure(reinterpret_cast<void*&>(ctx->binfo), ctx->binfo_bytes, (size_t)count * sizeof(int) + 256)))
+    return rc;
   ctx->have_times = false;
   ctx->info_dev = nullptr;
+  hipStream_t s = ctx->stream;
+  ZQ_HIP_CHECK(hipMemsetAsync(ctx->binfo, 0, (size_t)count * sizeof(int), s));
   if ((rc = zq::stage_begin(ctx->stage))) return rc;
-  int status = ZQ_OK;
-  for (size_t g = 0; g < groups.size() && !rc; ++g) {
-    std::vector<int> ginfo(info ? groups[g].size() : 0);
-    rc = solve_group(ctx, groups[g], gnb[g], vectors, info ? ginfo.data() : nullptr);
-    if (!rc && info) {
-      for (size_t k = 0; k < ginfo.size(); ++k) {
-        info[gidx[g][k]] = ginfo[k];
-        if (ginfo[k] != 0 && status == ZQ_OK) status = 1;
-      }
-    }
-  }
-  const int rc2 = zq::stage_end(ctx->stage, ctx->stream);
+  for (size_t g = 0; g < G.probs.size() && !rc; ++g) rc = solve_group(ctx, G.probs[g], G.nb[g], vectors);
+  const int rc2 = zq::stage_end(ctx->stage, s);
   if (rc) return rc;
   if (rc2) return rc2;
-  return status;
+  if (info_async)
+    ZQ_HIP_CHECK(hipMemcpyAsync(info_async, ctx->binfo, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (!info) return ZQ_OK;
+  ZQ_HIP_CHECK(hipMemcpyAsync(info, ctx->binfo, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, s));
+  ZQ_HIP_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < count; ++i)
+    if (info[i] != 0) return 1;
+  return ZQ_OK;
 }
 
 int check_batch(const zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
@@ -571,6 +574,7 @@ int zq_destroy(zq_context* c) {
   if (c->ws) (void)hipFree(c->ws);
   if (c->dD) (void)hipFree(c->dD);
   if (c->deig) (void)hipFree(c->deig);
+  if (c->binfo) (void)hipFree(c->binfo);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->prof.pool) (void)hipEventDestroy(e);
@@ -637,7 +641,7 @@ int zq_zquatev_batched_device(zq_context* c, int count, const int* n2, double* c
   if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -7;
   if (count == 0) return ZQ_OK;
   if ((rc = with_device(c))) return rc;
-  return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', info);
+  return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', info, nullptr);
 }
 
 int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
@@ -667,7 +671,7 @@ int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D
                                     hipMemcpyHostToDevice, s));
   }
   std::vector<int> inf(count, 0);
-  rc = solve_batched(c, count, n2, dptr.data(), dld.data(), eptr.data(), true, inf.data());
+  rc = solve_batched(c, count, n2, dptr.data(), dld.data(), eptr.data(), true, inf.data(), nullptr);
   if (rc < 0) return rc;
   for (int i = 0; i < count; ++i) {
     if (n2[i] == 0) continue;
@@ -797,3 +801,21 @@ int zquatev(const int n2, std::complex<double>* const D, const int nld2, double*
   return zq_zquatev(n2, reinterpret_cast<double*>(D), nld2, eig);
 }
 }  // namespace ts
+
+namespace zq {
+int batched_async(zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                  int jobz, int* info_async) {
+  int rc = check_batch(c, count, n2, D, ld2, eig, nullptr);
+  if (rc) return rc;
+  if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -7;
+  if (count == 0) return ZQ_OK;
+  if ((rc = with_device(c))) return rc;
+  return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', nullptr, info_async);
+}
+
+size_t batched_workspace(const zq_context* c, int count, const int* n2, int jobz) {
+  return make_groups(c, count, n2, nullptr, nullptr, nullptr, jobz == 'V' || jobz == 'v').need;
+}
+
+hipStream_t context_stream(const zq_context* c) { return c->stream; }
+}  // namespace zq

@@ -85,6 +85,18 @@ __global__ void __launch_bounds__(256) k_scatter_eig_b(const BatchProb* __restri
   if (i < n) P[blockIdx.z].eig[i] = wall[off + i];
 }
 
+__global__ void k_scatter_info_b(int nprob, const BatchProb* __restrict__ P, const int* __restrict__ info,
+                                 int* __restrict__ info_all) {
+  const int z = blockIdx.x * 256 + threadIdx.x;
+  if (z < nprob) info_all[P[z].idx] = info[z];
+}
+
+int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* info_all, hipStream_t s) {
+  if (nprob <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_scatter_info_b, dim3((nprob + 255) / 256), dim3(256), 0, s, nprob, P, info, info_all);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
 int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s) {
   if (nprob <= 0 || L <= 0) return ZQI_OK;
   hipLaunchKernelGGL(k_load_left_b, dim3((L + 255) / 256, L, nprob), dim3(256), 0, s, P, M, ld, zb);

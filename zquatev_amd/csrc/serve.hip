@@ -1,0 +1,274 @@
+// Serving loop: independent solve requests arrive one at a time and are
+// served with continuous batching.
+//
+// The reference is one synchronous call per matrix (ts::zquatev,
+// zquatev.cc:42-100); a caller with a stream of requests loops over it.
+// Here zq_server_submit only queues a request (device pointers, as
+// zq_zquatev_device takes them).  A scheduler thread takes whatever has
+// arrived (FIFO, one jobz per batch, at most max_batch problems and at most
+// max_bytes of workspace) and runs it as one batched call on a free lane; a
+// lane is a context with its own stream and resident workspace, so batches on
+// different lanes overlap on the device.  A completion thread watches each
+// batch's event and stamps its requests; zq_server_poll hands the records
+// out.  Results are those of the batched call, so each request's output is
+// bitwise that of its own zq_zquatev_device call with the same panel width.
+#include "zq_internal.h"
+#include "zq.h"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+  unsigned long long ticket;
+  int n2, ld2, jobz;
+  double* D;
+  double* eig;
+  double t_submit;
+};
+
+struct Batch {
+  std::vector<Req> reqs;
+  int lane = -1;
+  int rc = 0;
+  double t_start = 0.0;
+};
+
+}  // namespace
+
+struct zq_server {
+  int device = 0;
+  int max_batch = 64;
+  size_t max_bytes = 0;
+  std::vector<zq_context*> lanes;
+  std::vector<hipEvent_t> lane_ev;
+  std::vector<int*> lane_info;       // pinned, max_batch ints per lane
+  std::vector<int> lane_busy;
+  std::mutex mu;
+  std::condition_variable cv_work;   // scheduler: requests or a free lane
+  std::condition_variable cv_fly;    // completion thread: a batch in flight
+  std::condition_variable cv_done;   // drain
+  std::deque<Req> pending;
+  std::deque<Batch> inflight;
+  std::deque<zq_completion> done;
+  unsigned long long next_ticket = 1;
+  long outstanding = 0;
+  bool stop = false;
+  std::thread sched, compl_;
+  Clock::time_point t0;
+
+  double now_us() const { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
+
+  int free_lane() const {
+    for (size_t k = 0; k < lane_busy.size(); ++k)
+      if (!lane_busy[k]) return (int)k;
+    return -1;
+  }
+
+  void run_scheduler() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_work.wait(lk, [&] { return stop || (!pending.empty() && free_lane() >= 0); });
+      if (pending.empty() || free_lane() < 0) {
+        if (stop) break;
+        continue;
+      }
+      const int L = free_lane();
+      lane_busy[L] = 1;
+      Batch b;
+      b.lane = L;
+      const int jobz = pending.front().jobz;
+      std::vector<int> n2s;
+      for (auto it = pending.begin(); it != pending.end() && (int)b.reqs.size() < max_batch;) {
+        if (it->jobz != jobz) {
+          ++it;
+          continue;
+        }
+        n2s.push_back(it->n2);
+        if (!b.reqs.empty() && max_bytes > 0 &&
+            zq::batched_workspace(lanes[L], (int)n2s.size(), n2s.data(), jobz) > max_bytes) {
+          n2s.pop_back();
+          break;
+        }
+        b.reqs.push_back(*it);
+        it = pending.erase(it);
+      }
+      lk.unlock();
+      const int cnt = (int)b.reqs.size();
+      std::vector<int> ld2(cnt);
+      std::vector<double*> D(cnt), eig(cnt);
+      for (int k = 0; k < cnt; ++k) {
+        ld2[k] = b.reqs[k].ld2;
+        D[k] = b.reqs[k].D;
+        eig[k] = b.reqs[k].eig;
+      }
+      b.t_start = now_us();
+      b.rc = zq::batched_async(lanes[L], cnt, n2s.data(), D.data(), ld2.data(), eig.data(), jobz, lane_info[L]);
+      if (hipEventRecord(lane_ev[L], zq::context_stream(lanes[L])) != hipSuccess && b.rc == 0) b.rc = ZQ_ERR_HIP;
+      lk.lock();
+      inflight.push_back(std::move(b));
+      cv_fly.notify_one();
+    }
+  }
+
+  void run_completion() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_fly.wait(lk, [&] { return stop || !inflight.empty(); });
+      if (inflight.empty()) {
+        if (stop) break;
+        continue;
+      }
+      // find a finished batch (lanes may finish out of launch order)
+      int hit = -1;
+      for (size_t k = 0; k < inflight.size() && hit < 0; ++k) {
+        const hipError_t q = hipEventQuery(lane_ev[inflight[k].lane]);
+        if (q == hipSuccess) hit = (int)k;
+        else if (q != hipErrorNotReady) {
+          if (inflight[k].rc == 0) inflight[k].rc = ZQ_ERR_HIP;
+          hit = (int)k;
+        }
+      }
+      if (hit < 0) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+        continue;
+      }
+      Batch b = std::move(inflight[hit]);
+      inflight.erase(inflight.begin() + hit);
+      const double t = now_us();
+      const int* info = lane_info[b.lane];
+      for (size_t k = 0; k < b.reqs.size(); ++k) {
+        const Req& r = b.reqs[k];
+        zq_completion c;
+        c.ticket = r.ticket;
+        c.info = b.rc != 0 ? b.rc : info[k];
+        c.batch_size = (int)b.reqs.size();
+        c.lane = b.lane;
+        c.t_submit_us = r.t_submit;
+        c.t_start_us = b.t_start;
+        c.t_done_us = t;
+        done.push_back(c);
+      }
+      outstanding -= (long)b.reqs.size();
+      lane_busy[b.lane] = 0;
+      cv_work.notify_one();
+      cv_done.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+int zq_server_create(zq_server** out, int device, int lanes, int max_batch, double max_batch_gb) {
+  if (!out) return ZQ_ERR_ARG;
+  *out = nullptr;
+  if (lanes < 1 || lanes > 8) return -3;
+  if (max_batch < 1) return -4;
+  if (max_batch_gb < 0.0) return -5;
+  zq_server* s = new zq_server();
+  s->device = device;
+  s->max_batch = max_batch;
+  s->max_bytes = (size_t)(max_batch_gb * 1073741824.0);
+  s->t0 = Clock::now();
+  int rc = ZQ_OK;
+  for (int k = 0; k < lanes && rc == ZQ_OK; ++k) {
+    zq_context* c = nullptr;
+    rc = zq_create(&c, device);
+    if (rc) break;
+    s->lanes.push_back(c);
+    hipEvent_t e = nullptr;
+    void* h = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(&h, (size_t)max_batch * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      if (e) (void)hipEventDestroy(e);
+      rc = ZQ_ERR_HIP;
+      break;
+    }
+    s->lane_ev.push_back(e);
+    s->lane_info.push_back(static_cast<int*>(h));
+    s->lane_busy.push_back(0);
+  }
+  if (rc) {
+    for (auto c : s->lanes) zq_destroy(c);
+    for (auto e : s->lane_ev) (void)hipEventDestroy(e);
+    for (auto h : s->lane_info) (void)hipHostFree(h);
+    delete s;
+    return rc;
+  }
+  s->sched = std::thread([s] { s->run_scheduler(); });
+  s->compl_ = std::thread([s] { s->run_completion(); });
+  *out = s;
+  return ZQ_OK;
+}
+
+int zq_server_submit(zq_server* s, int n2, double* dD, int ld2, double* deig, int jobz, unsigned long long* ticket) {
+  if (!s) return ZQ_ERR_ARG;
+  if (n2 < 0 || (n2 & 1)) return -2;
+  if (n2 > 0 && !dD) return -3;
+  if (ld2 < (n2 > 1 ? n2 : 1)) return -4;
+  if (n2 > 0 && !deig) return -5;
+  if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -6;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const unsigned long long t = s->next_ticket++;
+  if (ticket) *ticket = t;
+  const double now = s->now_us();
+  if (n2 == 0) {
+    s->done.push_back(zq_completion{t, 0, 0, -1, now, now, now});
+    return ZQ_OK;
+  }
+  s->pending.push_back(Req{t, n2, ld2, jobz, dD, deig, now});
+  ++s->outstanding;
+  s->cv_work.notify_one();
+  return ZQ_OK;
+}
+
+int zq_server_poll(zq_server* s, zq_completion* out, int max) {
+  if (!s || (max > 0 && !out)) return ZQ_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int k = 0;
+  while (k < max && !s->done.empty()) {
+    out[k++] = s->done.front();
+    s->done.pop_front();
+  }
+  return k;
+}
+
+int zq_server_drain(zq_server* s) {
+  if (!s) return ZQ_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+  return ZQ_OK;
+}
+
+double zq_server_clock_us(zq_server* s) { return s ? s->now_us() : 0.0; }
+
+int zq_server_destroy(zq_server* s) {
+  if (!s) return ZQ_OK;
+  zq_server_drain(s);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  s->cv_fly.notify_all();
+  s->sched.join();
+  s->compl_.join();
+  for (auto c : s->lanes) zq_destroy(c);
+  for (auto e : s->lane_ev) (void)hipEventDestroy(e);
+  for (auto h : s->lane_info) (void)hipHostFree(h);
+  delete s;
+  return ZQ_OK;
+}
+
+}  // extern "C"

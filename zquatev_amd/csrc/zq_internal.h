@@ -354,7 +354,7 @@ struct BatchProb {
   int ld2;
   int n;
   int off;
-  int pad;
+  int idx;   // position in the caller's arrays
 };
 int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s);
 int store_vectors_batched(int nprob, int L, const BatchProb* P, const quat* X, int ldx, ZB zb, hipStream_t s);
@@ -362,6 +362,8 @@ int store_vectors_batched(int nprob, int L, const BatchProb* P, const quat* X, i
 int gather_tridiag_batched(int nprob, int L, const BatchProb* P, const double* d, const quat* e, ZB zb, double* dall,
                            double* eall, hipStream_t s);
 int scatter_eig_batched(int nprob, int L, const BatchProb* P, const double* wall, hipStream_t s);
+// info_all[P[z].idx] = info[z]
+int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* info_all, hipStream_t s);
 int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s);
 int store_vectors(int n, const quat* X, int ldx, double* D, int ld2, hipStream_t s);
 int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
@@ -403,4 +405,16 @@ int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* k
                  const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s);
 int mfma_probe(const double* a, const double* b, double* out, hipStream_t s);
 
+// api.hip hooks for the serving loop (serve.hip)
+}  // namespace zq
+struct zq_context;
+namespace zq {
+// zq_zquatev_batched_device without waiting: each problem's status is copied
+// into info_async (pinned host, count ints), valid once the context's stream
+// has passed the call.
+int batched_async(zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
+                  int jobz, int* info_async);
+// device workspace bytes such a batch needs on c
+size_t batched_workspace(const zq_context* c, int count, const int* n2, int jobz);
+hipStream_t context_stream(const zq_context* c);
 }  // namespace zq
