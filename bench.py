@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--loop-seconds", type=float, default=10.0, help="arrival window replayed through the plain loop")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--pool", type=int, default=1024)
-    ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-batch-gb", type=float, default=24.0)
     ap.add_argument("--batch-count", type=int, default=512)

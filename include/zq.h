@@ -95,9 +95,10 @@ int zq_zquatev_batched_device(zq_context* ctx, int count, const int* n2, double*
  * No counterpart in the reference (one synchronous ts::zquatev call per
  * matrix, zquatev.cc:42-100).  Requests are queued by zq_server_submit and
  * served by continuous batching: a scheduler thread batches whatever has
- * arrived (FIFO, same jobz, at most max_batch problems and max_batch_gb of
- * workspace) onto one of `lanes` contexts (own stream, resident workspace);
- * batches on different lanes run concurrently.  Each request's result is
+ * arrived (the oldest request and those after it of the same jobz and size
+ * class -- 2n up to 384, up to 1280, larger -- at most max_batch problems and
+ * max_batch_gb of workspace) onto one of `lanes` contexts (own stream,
+ * resident workspace); batches on different lanes run concurrently.  Each request's result is
  * bitwise that of its own zq_zquatev_device call.  Device pointers stay
  * owned by the caller and must stay valid until the request's completion
  * record has been returned by zq_server_poll.  Times are microseconds of a

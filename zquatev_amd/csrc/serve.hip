@@ -5,8 +5,15 @@
 // zquatev.cc:42-100); a caller with a stream of requests loops over it.
 // Here zq_server_submit only queues a request (device pointers, as
 // zq_zquatev_device takes them).  A scheduler thread takes whatever has
-// arrived (FIFO, one jobz per batch, at most max_batch problems and at most
-// max_bytes of workspace) and runs it as one batched call on a free lane; a
+// arrived and runs it as one batched call on a free lane: the oldest request
+// picks the batch's jobz and size class (half-dimension up to 192, up to 640,
+// larger: a lockstep batch runs as long as its largest member, so small
+// requests are not held behind large ones), then further requests of that
+// jobz and class join in arrival order, up to max_batch problems and
+// max_bytes of workspace.  Large-class batches occupy at most all lanes but
+// one, so small requests always find a lane soon.  Each lane has its own
+// thread that enqueues its batch (a large batch is thousands of launches),
+// so lanes never wait for each other's host work.  A
 // lane is a context with its own stream and resident workspace, so batches on
 // different lanes overlap on the device.  A completion thread watches each
 // batch's event and stamps its requests; zq_server_poll hands the records
@@ -37,6 +44,7 @@ struct Req {
 struct Batch {
   std::vector<Req> reqs;
   int lane = -1;
+  int cls = 0;
   int rc = 0;
   double t_start = 0.0;
 };
@@ -51,6 +59,11 @@ struct zq_server {
   std::vector<hipEvent_t> lane_ev;
   std::vector<int*> lane_info;       // pinned, max_batch ints per lane
   std::vector<int> lane_busy;
+  std::vector<int> lane_cls;         // size class of the lane's batch, -1 when free
+  std::vector<Batch> lane_job;       // batch handed to the lane's thread
+  std::vector<int> lane_has_job;
+  std::vector<std::thread> lane_threads;
+  std::condition_variable cv_lane;   // lane threads: a batch was handed over
   std::mutex mu;
   std::condition_variable cv_work;   // scheduler: requests or a free lane
   std::condition_variable cv_fly;    // completion thread: a batch in flight
@@ -65,6 +78,8 @@ struct zq_server {
   Clock::time_point t0;
 
   double now_us() const { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
+
+  static int size_class(int n2) { return n2 <= 384 ? 0 : (n2 <= 1280 ? 1 : 2); }
 
   int free_lane() const {
     for (size_t k = 0; k < lane_busy.size(); ++k)
@@ -82,13 +97,28 @@ struct zq_server {
         continue;
       }
       const int L = free_lane();
+      // the head request picks the batch; large-class batches may hold at
+      // most lanes - 1 lanes, so one lane always stays open to small requests
+      int big_busy = 0;
+      for (int c : lane_cls) big_busy += c == 2 ? 1 : 0;
+      auto head = pending.begin();
+      if (lanes.size() > 1 && big_busy + 1 >= (int)lanes.size()) {
+        while (head != pending.end() && size_class(head->n2) == 2) ++head;
+        if (head == pending.end()) {
+          // only large requests wait: let the running batches finish first
+          cv_work.wait_for(lk, std::chrono::microseconds(200));
+          continue;
+        }
+      }
       lane_busy[L] = 1;
       Batch b;
       b.lane = L;
-      const int jobz = pending.front().jobz;
+      const int jobz = head->jobz;
+      const int cls = size_class(head->n2);
+      b.cls = cls;
       std::vector<int> n2s;
-      for (auto it = pending.begin(); it != pending.end() && (int)b.reqs.size() < max_batch;) {
-        if (it->jobz != jobz) {
+      for (auto it = head; it != pending.end() && (int)b.reqs.size() < max_batch;) {
+        if (it->jobz != jobz || size_class(it->n2) != cls) {
           ++it;
           continue;
         }
@@ -101,17 +131,37 @@ struct zq_server {
         b.reqs.push_back(*it);
         it = pending.erase(it);
       }
+      lane_cls[L] = cls;
+      lane_job[L] = std::move(b);
+      lane_has_job[L] = 1;
+      cv_lane.notify_all();
+    }
+  }
+
+  void run_lane(int L) {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_lane.wait(lk, [&] { return stop || lane_has_job[L]; });
+      if (!lane_has_job[L]) {
+        if (stop) break;
+        continue;
+      }
+      Batch b = std::move(lane_job[L]);
+      lane_has_job[L] = 0;
       lk.unlock();
       const int cnt = (int)b.reqs.size();
-      std::vector<int> ld2(cnt);
+      std::vector<int> n2s(cnt), ld2(cnt);
       std::vector<double*> D(cnt), eig(cnt);
       for (int k = 0; k < cnt; ++k) {
+        n2s[k] = b.reqs[k].n2;
         ld2[k] = b.reqs[k].ld2;
         D[k] = b.reqs[k].D;
         eig[k] = b.reqs[k].eig;
       }
       b.t_start = now_us();
-      b.rc = zq::batched_async(lanes[L], cnt, n2s.data(), D.data(), ld2.data(), eig.data(), jobz, lane_info[L]);
+      b.rc = zq::batched_async(lanes[L], cnt, n2s.data(), D.data(), ld2.data(), eig.data(), b.reqs[0].jobz,
+                               lane_info[L]);
       if (hipEventRecord(lane_ev[L], zq::context_stream(lanes[L])) != hipSuccess && b.rc == 0) b.rc = ZQ_ERR_HIP;
       lk.lock();
       inflight.push_back(std::move(b));
@@ -162,6 +212,7 @@ struct zq_server {
       }
       outstanding -= (long)b.reqs.size();
       lane_busy[b.lane] = 0;
+      lane_cls[b.lane] = -1;
       cv_work.notify_one();
       cv_done.notify_all();
     }
@@ -198,7 +249,10 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
     s->lane_ev.push_back(e);
     s->lane_info.push_back(static_cast<int*>(h));
     s->lane_busy.push_back(0);
+    s->lane_cls.push_back(-1);
+    s->lane_has_job.push_back(0);
   }
+  s->lane_job.resize(s->lanes.size());
   if (rc) {
     for (auto c : s->lanes) zq_destroy(c);
     for (auto e : s->lane_ev) (void)hipEventDestroy(e);
@@ -206,6 +260,7 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
     delete s;
     return rc;
   }
+  for (int k = 0; k < (int)s->lanes.size(); ++k) s->lane_threads.emplace_back([s, k] { s->run_lane(k); });
   s->sched = std::thread([s] { s->run_scheduler(); });
   s->compl_ = std::thread([s] { s->run_completion(); });
   *out = s;
@@ -262,7 +317,9 @@ int zq_server_destroy(zq_server* s) {
   }
   s->cv_work.notify_all();
   s->cv_fly.notify_all();
+  s->cv_lane.notify_all();
   s->sched.join();
+  for (auto& t : s->lane_threads) t.join();
   s->compl_.join();
   for (auto c : s->lanes) zq_destroy(c);
   for (auto e : s->lane_ev) (void)hipEventDestroy(e);
