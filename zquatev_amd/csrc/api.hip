@@ -180,7 +180,6 @@ void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   p->e = b.take<quat>(nmax);
   p->ea = b.take<double>(nmax);
   p->rw.W = b.take<quat>((size_t)n * nb);
-  p->rw.ucur = b.take<quat>(nmax);
   p->rw.Prow = b.take<quat>((size_t)cdiv(nmax, 32) * n);
   p->rw.Pcol = b.take<quat>((size_t)cdiv(nmax, 64) * n);
   p->rw.TS = b.take<quat>((size_t)2 * nb);

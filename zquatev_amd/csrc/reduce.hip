@@ -189,42 +189,43 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
   return R;
 }
 
-// u_i from the raw column entry x_i = M[i, c]
-__device__ __forceinline__ quat u_from(const quat& x, int c, int i, const Refl& R) {
-  return (i == c + 1) ? R.u1 : qscale(x, R.g);
-}
 
-// Hermitian matvec y = M[s0:, s0:] u (s0 = c + 1) over the lower triangle.
+// Hermitian matvec over the lower triangle of the trailing matrix
+// M[s0:, s0:] (s0 = c + 1), applied to the raw updated column x = M[s0:, c]
+// rather than to the reflector u: u = gamma x + (u1 - gamma x1) e_1, so
+//   M u = gamma (M x) + (u1 - gamma x1) M[s0:, s0],
+// which k_wreduce forms row by row.  The tile blocks therefore never wait for
+// the reflector (no norm reduction on their path).
 // Block (g, J) reads rows [s0 + SG g, +SG) x columns [s0 + SW J, +SW) (every
 // lower-triangle element exactly once over the grid); wave w of 8 takes 4 of
-// the 32 columns, one row per lane.  The matrix loads are issued first, so
-// the reflector set-up (tail-norm reduction, x_1) runs under their latency.
-// Row sums over the 32 columns (summed across the waves in LDS) go to
-// Prow[J][row]; column sums over the 64 rows (a VALU-only halving butterfly,
-// DPP and permlane swaps) go to Pcol[g][col]; then
-//   y(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
+// the 32 columns, one row per lane, all loads issued up front.  Row sums over
+// the 32 columns (summed across the waves in LDS) go to Prow[J][row]; column
+// sums over the 64 rows (a VALU-only halving butterfly, DPP and permlane
+// swaps) go to Pcol[g][col]; then
+//   (M x)(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
 // every (slot, row) written exactly once, so the sum is deterministic.
 // Blocks are enumerated row-group major: group g has 2 (g + 1) strips.
-// The ceil(8 j / 64) blocks after the G (G + 1) tiles finish the tall-skinny
-// products TS = [W^* u | V^* u]: they add the per-block partials of W^* x and
-// V^* x that k_col_update left in TSP (8 waves over the partial index, then
-// the 8 wave sums, in a fixed order), scale by the reflector's gamma and add
-// the row c+1 term, u_{c+1} = u1.
+// The max(1, ceil(8 j / 64)) blocks after the G (G + 1) tiles compute the
+// reflector from the tail-norm partials (the first of them publishes gamma,
+// u1, x1 in scal and d[c], e[c]) and finish the tall-skinny products
+// TS = [W^* u | V^* u]: they add the per-block partials of W^* x and V^* x
+// that k_col_update left in TSP (8 waves over the partial index, then the 8
+// wave sums, in a fixed order), scale by gamma and add the row s0 term.
 __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                               const quat* __restrict__ W, int ldw, const double* __restrict__ part,
                                               int nparts, double* __restrict__ scal, double* __restrict__ d,
                                               quat* __restrict__ e, quat* __restrict__ Prow, quat* __restrict__ Pcol,
-                                              int ldp, quat* __restrict__ ucur, quat* __restrict__ TS, int nb,
-                                              const double* __restrict__ TSP, int ldtsp, ZB zb) {
+                                              int ldp, quat* __restrict__ TS, int nb, const double* __restrict__ TSP,
+                                              int ldtsp, ZB zb) {
   const int NT = j > 0 ? (8 * j + 63) / 64 : 0;
+  const int NF = NT > 0 ? NT : 1;
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     nparts = (n - c + RQ - 1) / RQ;
     const int Gz = (n - c - 1 + SG - 1) / SG;
-    if ((int)blockIdx.x >= Gz * (Gz + 1) + NT) return;
+    if ((int)blockIdx.x >= Gz * (Gz + 1) + NF) return;
     M = zoff(M, zb); W = zoff(W, zb); part = zoff(part, zb); scal = zoff(scal, zb); d = zoff(d, zb);
-    e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); ucur = zoff(ucur, zb); TS = zoff(TS, zb);
-    TSP = zoff(TSP, zb);
+    e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb); TSP = zoff(TSP, zb);
   }
   __shared__ double sh[SYT / 64];
   __shared__ quat uJ[SW];
@@ -233,7 +234,57 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int b = blockIdx.x;
-  // block -> (g, J): g (g + 1) <= b < (g + 1) (g + 2)
+  const int Gt = (n - s0 + SG - 1) / SG;
+  const int bt = b - Gt * (Gt + 1);
+  if (bt >= 0) {
+    // ---- finishing block: reflector, scalars, tall-skinny products
+    const bool has_ts = bt < NT;
+    double tsacc = 0.0;
+    if (has_ts) {
+      const int eidx = 64 * bt + lane;
+#pragma unroll 8
+      for (int k = w; k < nparts; k += 8) tsacc += TSP[(size_t)k * ldtsp + eidx];
+    }
+    const quat x1 = qload(M + s0 + (size_t)c * ld);
+    double tail = 0.0;
+    for (int k = tid; k < nparts; k += SYT) tail += part[k];
+    tail = block_sum(tail, sh);
+    quat alpha;
+    const Refl R = reflector(x1, tail, &alpha);
+    if (bt == 0 && tid == 0) {
+      scal[0] = R.g;
+      scal[1] = R.u1.ar; scal[2] = R.u1.ai; scal[3] = R.u1.br; scal[4] = R.u1.bi;
+      scal[5] = x1.ar; scal[6] = x1.ai; scal[7] = x1.br; scal[8] = x1.bi;
+      d[c] = M[c + (size_t)c * ld].ar;
+      qstore(e + c, alpha);
+    }
+    if (has_ts) {
+      __shared__ double tsr[8][64];
+      tsr[w][lane] = tsacc;
+      __syncthreads();
+      if (tid < 16) {
+        // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
+        const int e0 = 64 * bt + 4 * tid;
+        const int l = e0 >> 3, side = (e0 >> 2) & 1;
+        if (l < j) {
+          double sc[4];
+#pragma unroll
+          for (int comp = 0; comp < 4; ++comp) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
+            sc[comp] = t;
+          }
+          const quat rowv = side == 0 ? qload(W + s0 + (size_t)l * ldw) : qload(M + s0 + (size_t)(p + l) * ld);
+          quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
+          qfma_cl(t, rowv, R.u1);
+          qstore(TS + (size_t)side * nb + l, t);
+        }
+      }
+    }
+    return;
+  }
+  // ---- tile block: block -> (g, J) with g (g + 1) <= b < (g + 1) (g + 2)
   int g = (int)((sqrt(4.0 * b + 1.0) - 1.0) * 0.5);
   while ((g + 1) * (g + 2) <= b) ++g;
   while (g * (g + 1) > b) --g;
@@ -250,31 +301,8 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   }
   const quat xa = qload(M + rac + (size_t)c * ld);
   const quat xj = qload(M + min(C0 + (tid & (SW - 1)), n - 1) + (size_t)c * ld);
-  const quat x1 = qload(M + (c + 1) + (size_t)c * ld);
-  // tall-skinny finishing block: partial sums of entries 64 bt + lane, wave w taking every 8th block
-  const int Gt = (n - s0 + SG - 1) / SG;
-  const int bt = b - Gt * (Gt + 1);
-  const bool has_ts = bt >= 0 && bt < NT;
-  double tsacc = 0.0;
-  if (has_ts) {
-    const int eidx = 64 * bt + lane;
-#pragma unroll 8
-    for (int k = w; k < nparts; k += 8) tsacc += TSP[(size_t)k * ldtsp + eidx];
-  }
-  double tail = 0.0;
-  for (int k = tid; k < nparts; k += SYT) tail += part[k];
-  tail = block_sum(tail, sh);
-  quat alpha;
-  const Refl R = reflector(x1, tail, &alpha);
-  if (b == 0 && tid == 0) {
-    scal[0] = R.g;
-    scal[1] = R.u1.ar; scal[2] = R.u1.ai; scal[3] = R.u1.br; scal[4] = R.u1.bi;
-    d[c] = M[c + (size_t)c * ld].ar;
-    qstore(e + c, alpha);
-  }
-  if (tid < SW) uJ[tid] = (C0 + tid < n) ? u_from(xj, c, C0 + tid, R) : qzero();
-  const quat ua = (ra < n) ? u_from(xa, c, ra, R) : qzero();
-  if (active && J == 0 && w == 0 && ra < n) qstore(ucur + ra, ua);
+  if (tid < SW) uJ[tid] = (C0 + tid < n) ? xj : qzero();
+  const quat ua = (ra < n) ? xa : qzero();
   __syncthreads();
   if (active) {
     quat acc = qzero();
@@ -308,46 +336,23 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
-  if (has_ts) {
-    __shared__ double tsr[8][64];
-    tsr[w][lane] = tsacc;
-    __syncthreads();
-    if (tid < 16) {
-      // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
-      const int e0 = 64 * bt + 4 * tid;
-      const int l = e0 >> 3, side = (e0 >> 2) & 1;
-      if (l < j) {
-        double sc[4];
-#pragma unroll
-        for (int comp = 0; comp < 4; ++comp) {
-          double t = 0.0;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
-          sc[comp] = t;
-        }
-        const quat rowv = side == 0 ? qload(W + (c + 1) + (size_t)l * ldw) : qload(M + (c + 1) + (size_t)(p + l) * ld);
-        quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
-        qfma_cl(t, rowv, R.u1);
-        qstore(TS + (size_t)side * nb + l, t);
-      }
-    }
-  }
 }
 
-// p = y - V (W^* u) - W (V^* u), y from the matvec partials; stored in W[:, j].  Also stores u into
+// p = y - V (W^* u) - W (V^* u) with y = M u = gamma (M x) + M[:, s0] (u1 - gamma x1),
+// (M x) from the matvec partials; stored in W[:, j].  Also stores u into
 // M[c+1:, c], partial u^* p into KP, and (block 0) column j of the strict
 // upper part of G = V^* V (the T factor is formed from G after the panel).
 template <int KPT>
 __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                                 quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                                 const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
-                                                int nb, const quat* __restrict__ ucur,
+                                                int nb, const double* __restrict__ scal,
                                                 double* __restrict__ KP, quat* __restrict__ Gm, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     if (c + 1 + (int)blockIdx.x * RQ >= n) return;
     M = zoff(M, zb); W = zoff(W, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb);
-    ucur = zoff(ucur, zb); KP = zoff(KP, zb); Gm = zoff(Gm, zb);
+    scal = zoff(scal, zb); KP = zoff(KP, zb); Gm = zoff(Gm, zb);
   }
   __shared__ quat a[NBMAX], b[NBMAX];
   __shared__ quat red[16][RQ];
@@ -365,7 +370,12 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
     vv[k] = qload(M + ic + (size_t)(p + l) * ld);
     ww[k] = qload(W + ic + (size_t)l * ldw);
   }
-  const quat ui = qload(ucur + ic);
+  const quat xi = qload(M + ic + (size_t)c * ld);
+  quat ms0 = qload(M + ic + (size_t)s0 * ld);
+  if (ic == s0) ms0 = quat{ms0.ar, 0.0, 0.0, 0.0};   // Hermitian diagonal: real
+  const double gam = scal[0];
+  const quat u1 = quat{scal[1], scal[2], scal[3], scal[4]};
+  const quat x1 = quat{scal[5], scal[6], scal[7], scal[8]};
   quat yi = qzero();
   {
     const int li = ic - s0;
@@ -383,7 +393,7 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
   if (blockIdx.x == 0) {
     for (int rr = tid; rr < nb; rr += RB) qstore(Gm + rr + (size_t)j * nb, rr < j ? b[rr] : qzero());
   }
-  quat acc = yi;   // this thread's share of y(i) minus its share of the corrections
+  quat acc = qscale(yi, gam);   // this thread's share of y(i) minus its share of the corrections
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int l = q + 16 * k;
@@ -396,9 +406,10 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
   __syncthreads();
   double kp = 0.0;
   if (q == 0 && i < n) {
-    quat pv = qzero();
+    quat pv = qmul(ms0, qsub(u1, qscale(x1, gam)));
 #pragma unroll
     for (int k = 0; k < 16; ++k) pv = qadd(pv, red[k][r]);
+    const quat ui = (i == s0) ? u1 : qscale(xi, gam);
     qstore(W + i + (size_t)j * ldw, pv);
     qstore(M + i + (size_t)c * ld, ui);
     kp = ui.ar * pv.ar + ui.ai * pv.ai + ui.br * pv.br + ui.bi * pv.bi;
@@ -523,15 +534,15 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
                         ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 0);
+      const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
         launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
-                 w.scal, d, e, w.Prow, w.Pcol, n, w.ucur, w.TS, nb, w.TSP, w.ldtsp, ZB{nullptr, 0});
+                 w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, ZB{nullptr, 0});
       }
       const int nb2 = cdiv(sz, RQ);
-      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.ucur, w.KP, Gp,
+      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal, w.KP, Gp,
                      ZB{nullptr, 0});
       if (j == wdt - 1)
         hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2,
@@ -603,12 +614,12 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
                       zb);
     const int sz = L - c - 1;
     const int G = cdiv(sz, SG);
-    const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 0);
+    const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
     hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
-                       w.scal, d, e, w.Prow, w.Pcol, ld, w.ucur, w.TS, nb, w.TSP, w.ldtsp, zb);
+                       w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, zb);
     const int nb2 = cdiv(sz, RQ);
     quat* Gp = w.Gm + (size_t)(c / nb) * nb * nb;
-    launch_wreduce(kpt, dim3(nb2, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb, w.ucur,
+    launch_wreduce(kpt, dim3(nb2, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb, w.scal,
                    w.KP, Gp, zb);
     // w of the last column of a panel: every active problem at a panel end,
     // else those whose last column is c

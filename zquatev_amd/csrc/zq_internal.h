@@ -250,7 +250,6 @@ __device__ __forceinline__ T* zoff(T* p, const ZB& zb) {
 // Tf (per panel nb x nb compact-WY factors) and leaves u_c in M[c+1:, c].
 struct ReduceWork {
   quat* W;        // n x nb (global row index)
-  quat* ucur;     // n
   quat* Prow;     // matvec row partials: ceil(n/32) x n
   quat* Pcol;     // matvec column partials: ceil(n/64) x n
   quat* TS;       // tall-skinny products [W^* u | V^* u]: 2 x nb
