@@ -228,7 +228,6 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb); TSP = zoff(TSP, zb);
   }
   __shared__ double sh[SYT / 64];
-  __shared__ quat uJ[SW];
   __shared__ quat red[8][SG];
   const int s0 = c + 1;
   const int tid = threadIdx.x;
@@ -300,10 +299,11 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     qa[cc] = qload(M + rac + (size_t)cgc * ld);
   }
   const quat xa = qload(M + rac + (size_t)c * ld);
-  const quat xj = qload(M + min(C0 + (tid & (SW - 1)), n - 1) + (size_t)c * ld);
-  if (tid < SW) uJ[tid] = (C0 + tid < n) ? xj : qzero();
+  // x at this wave's 4 columns (the same address across the wave: one fetch each)
+  quat xc[4];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) xc[cc] = qload(M + min(C0 + 4 * w + cc, n - 1) + (size_t)c * ld);
   const quat ua = (ra < n) ? xa : qzero();
-  __syncthreads();
   if (active) {
     quat acc = qzero();
     double col[16];
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       const int cg = C0 + cl;
       quat a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
       if (ra == cg) a = quat{a.ar, 0.0, 0.0, 0.0};
-      qfma(acc, a, uJ[cl]);
+      qfma(acc, a, cg < n ? xc[cc] : qzero());
       quat cp = qzero();
       if (ra > cg) qfma_cl(cp, a, ua);
       col[4 * cc + 0] = cp.ar; col[4 * cc + 1] = cp.ai; col[4 * cc + 2] = cp.br; col[4 * cc + 3] = cp.bi;
