@@ -145,6 +145,9 @@ using zq::quat;
 
 constexpr int kNSplit = 8;
 
+// reflectors per back-transform block (a multiple of the panel width)
+inline int backtr_width(int n) { return n > 1024 ? 128 : 64; }
+
 struct Plan {
   quat* M;
   quat* X;
@@ -154,6 +157,8 @@ struct Plan {
   zq::ReduceWork rw;
   zq::StedcWork sw;
   zq::BacktrWork bw;
+  zq::QGemmTask* btasks;   // back-transform task tables (single solve)
+  zq::SumTask* bsums;
 };
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -188,16 +193,20 @@ void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   p->rw.KP = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.part = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.scal = b.take<double>(16);
-  p->rw.Tf = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);
-  p->rw.Gm = b.take<quat>((size_t)cdiv(nmax, nb) * nb * nb);
   if (vectors) {
     p->X = b.take<quat>(nn);
+    const int wbt = backtr_width(n);
+    const int nblk = cdiv(nmax - 1, wbt);
     p->bw.phi = b.take<quat>(nmax);
     p->bw.nsplit = kNSplit;
-    p->bw.ypart_slots = (size_t)kNSplit * nb * n;
-    p->bw.Ypart = b.take<quat>(p->bw.ypart_slots);
-    p->bw.Y = b.take<quat>((size_t)nb * n);
-    p->bw.Y2 = b.take<quat>((size_t)nb * n);
+    p->bw.wbt = wbt;
+    p->bw.part_slots = (size_t)kNSplit * wbt * (n + wbt);
+    p->bw.part = b.take<quat>(p->bw.part_slots);
+    p->bw.Y = b.take<quat>((size_t)wbt * n);
+    p->bw.G = b.take<quat>((size_t)nblk * wbt * wbt);
+    p->bw.T = b.take<quat>((size_t)nblk * wbt * wbt);
+    p->bw.vt_slots = zq::backtr_vt_slots(n, wbt);
+    p->bw.VT = b.take<quat>(p->bw.vt_slots);
   } else {
     p->X = nullptr;
   }
@@ -248,6 +257,12 @@ size_t layout_plan(int n, int nb, bool vectors, char* base, Plan* p) {
   Bump b{base};
   slab_layout(b, n, nb, vectors, p);
   stedc_layout(b, n, 1, n, vectors, &p->sw);
+  p->btasks = nullptr;
+  p->bsums = nullptr;
+  if (vectors) {
+    p->btasks = b.take<zq::QGemmTask>(zq::backtransform_batched_tasks(1, &n, backtr_width(n), kNSplit));
+    p->bsums = b.take<zq::SumTask>(zq::backtransform_batched_sums(1, &n, backtr_width(n)));
+  }
   return b.off + 256;
 }
 
@@ -305,16 +320,20 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
   if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
   ctx->info_dev = vectors ? p.sw.info : nullptr;
-  {
-    const int zero = 0;
-    p.sw.stage = &ctx->stage;
-    if ((rc = zq::stage_begin(ctx->stage))) return rc;
-    if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
-    if ((rc = zq::stage_end(ctx->stage, s))) return rc;
-  }
+  if ((rc = zq::stage_begin(ctx->stage))) return rc;
+  struct EndStage {
+    zq_context* c;
+    hipStream_t s;
+    ~EndStage() { (void)zq::stage_end(c->stage, s); }
+  } end_stage{ctx, s};
+  const int zero = 0;
+  p.sw.stage = &ctx->stage;
+  if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   if (vectors) {
-    if ((rc = zq::backtransform(n, nb, p.M, n, p.rw.Tf, p.e, p.sw.Z, n, p.X, n, p.bw, s))) return rc;
+    if ((rc = zq::backtransform_batched(1, &n, zq::ZB{nullptr, 0}, nullptr, p.M, n, p.e, p.sw.Z, n, p.X, n, p.bw,
+                                        p.btasks, p.bsums, ctx->stage, s)))
+      return rc;
     ZQ_HIP_CHECK(hipEventRecord(ctx->ev[4], s));
     if ((rc = zq::store_vectors(n, p.X, n, dD, ld2, s))) return rc;
   } else {
@@ -368,8 +387,8 @@ size_t batch_layout(int nprob, const int* nh, int nb, bool vectors, char* base, 
   stedc_layout(b, ntot, nprob, L, vectors, &bp->sw);
   bp->rtasks = b.take<zq::QGemmTask>(zq::reduce_batched_tasks(nprob, nh, nb));
   if (vectors) {
-    bp->btasks = b.take<zq::QGemmTask>(zq::backtransform_batched_tasks(nprob, nh, nb, kNSplit));
-    bp->sums = b.take<zq::SumTask>((size_t)nprob * (L > 1 ? cdiv(L - 1, nb) : 0) + 1);
+    bp->btasks = b.take<zq::QGemmTask>(zq::backtransform_batched_tasks(nprob, nh, backtr_width(L), kNSplit));
+    bp->sums = b.take<zq::SumTask>(zq::backtransform_batched_sums(nprob, nh, backtr_width(L)));
   }
   return b.off + 256;
 }
@@ -407,9 +426,8 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
     return rc;
   if ((rc = zq::scatter_eig_batched(nprob, L, bp.probs, bp.wall, s))) return rc;
   if (vectors) {
-    if ((rc = zq::backtransform_batched(nprob, nh.data(), zb, zrow.data(), bp.zrow, nb, bp.slab.M, L,
-                                        bp.slab.rw.Tf, bp.slab.e, bp.sw.Z, ntot, bp.slab.X, L, bp.slab.bw,
-                                        bp.btasks, bp.sums, st, s)))
+    if ((rc = zq::backtransform_batched(nprob, nh.data(), zb, bp.zrow, bp.slab.M, L, bp.slab.e, bp.sw.Z, ntot,
+                                        bp.slab.X, L, bp.slab.bw, bp.btasks, bp.sums, st, s)))
       return rc;
     if ((rc = zq::store_vectors_batched(nprob, L, bp.probs, bp.slab.X, L, zb, s))) return rc;
     if ((rc = zq::scatter_info_batched(nprob, bp.probs, bp.sw.info, ctx->binfo, s))) return rc;

@@ -2,13 +2,15 @@
 //
 // Replaces the reference's explicit Q accumulation (blocked.cc:477-544,
 // unblocked.cc:75-80, :121-126) followed by U = Q0 Z, V = Q1 Z
-// (zquatev.cc:86-90).  Q is never formed: the panel reflectors, kept in place
-// below the subdiagonal of M with their compact-WY factors T
-// (H_p ... H_{p+nb-1} = I - V T V^*), are applied to Phi Z panel by panel in
-// reverse order, each as three FP64 MFMA products:
-//   Y = V^* X[r0:, :]   (split over the long K dimension),
-//   Y2 = T Y,
-//   X[r0:, :] -= V Y2.
+// (zquatev.cc:86-90).  Q is never formed.  The reflectors u_c, kept in place
+// below the subdiagonal of M, are grouped in blocks of wbt consecutive ones
+// (independent of the reduction's panel width) with compact-WY factors,
+// H_p ... H_{p+w-1} = I - V T V^*, T = (I + triu(V^* V, 1))^{-1} (every tau is
+// 1 because u^* u = 2).  For all blocks at once: G = V^* V (split-K FP64
+// MFMA products and their sums), T, and VT = V T^*.  Then per block, last
+// block first, two MFMA products on X's rows r0 = p + 1 and below:
+//   Y = VT^* X[r0:, :]   (= T V^* X, split over the long dimension),
+//   X[r0:, :] -= V Y.
 // Phi is the diagonal of unit quaternions with Phi^* T_q Phi real (zhbtrd's
 // phase scaling for kd = 1, applied here to the quaternion subdiagonal); Z is
 // the real eigenvector matrix of the tridiagonal.
@@ -82,16 +84,7 @@ __global__ void __launch_bounds__(256) k_phase_apply(int n, const quat* __restri
   qstore(X + i + (size_t)j * ldx, qscale(qload(phi + i), z));
 }
 
-// Y[b, col] = sum_z Ypart[z][b, col]   (w x nn, ld w)
-__global__ void __launch_bounds__(256) k_sum_parts(int total, int nz, const quat* __restrict__ Yp, quat* __restrict__ Y) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= total) return;
-  quat s = qzero();
-  for (int z = 0; z < nz; ++z) s = qadd(s, qload(Yp + (size_t)z * total + i));
-  qstore(Y + i, s);
-}
-
-// the same sum for a list of tasks (task blockIdx.z)
+// out = sum_z parts[z] (each part `total` quaternions) for a list of tasks (task blockIdx.z)
 __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__ tasks) {
   const SumTask t = tasks[blockIdx.z];
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -101,61 +94,62 @@ __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__
   qstore(t.out + i, s);
 }
 
+// T = (I + triu(G, 1))^{-1} for each block: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
+// T[j, j] = 1, zero below (the compact-WY recurrence with every tau = 1).
+// One workgroup per block (blockIdx.x), problem blockIdx.z.
+__global__ void __launch_bounds__(256) k_tfactor(int n, int wbt, const quat* __restrict__ Gall,
+                                                 quat* __restrict__ Tall, ZB zb) {
+  const int blk = blockIdx.x;
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    Gall = zoff(Gall, zb); Tall = zoff(Tall, zb);
+  }
+  if (blk >= (n - 1 + wbt - 1) / wbt) return;
+  const int p = blk * wbt;
+  const int wdt = min(wbt, n - 1 - p);
+  const quat* G = Gall + (size_t)blk * wbt * wbt;
+  quat* T = Tall + (size_t)blk * wbt * wbt;
+  for (int jj = 0; jj < wbt; ++jj) {
+    for (int r = threadIdx.x; r < wbt; r += 256) {
+      quat acc = qzero();
+      if (jj < wdt && r < jj) {
+#pragma unroll 4
+        for (int l = r; l < jj; ++l) qfma(acc, qload(T + r + (size_t)l * wbt), qload(G + l + (size_t)jj * wbt));
+        acc = qscale(acc, -1.0);
+      } else if (r == jj) {
+        acc = quat{1.0, 0.0, 0.0, 0.0};
+      }
+      qstore(T + r + (size_t)jj * wbt, acc);
+    }
+    __syncthreads();
+  }
+}
+
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-int backtransform(int n, int nb, const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz,
-                  quat* X, int ldx, const BacktrWork& w, hipStream_t s) {
-  if (n <= 0) return ZQI_OK;
-  hipLaunchKernelGGL(k_phases, dim3(1), dim3(PH_T), 0, s, n, e, w.phi, ZB{nullptr, 0});
-  hipLaunchKernelGGL(k_phase_apply, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, w.phi, Z, ldz, X, ldx,
-                     ZB{nullptr, 0}, (const int*)nullptr);
-  if (hipGetLastError() != hipSuccess) return ZQI_ERR_HIP;
-  if (n == 1) return ZQI_OK;
-  const int npanel = cdiv(n - 1, nb);
-  for (int panel = npanel - 1; panel >= 0; --panel) {
-    const int p = panel * nb;
-    const int wdt = (nb < n - 1 - p) ? nb : (n - 1 - p);
-    const int r0 = p + 1;
-    const int mr = n - r0;
-    // V: rows r0.., cols p..p+wdt-1 of M; u_{p+t} lives in rows >= p+t+1, i.e. local row s >= t
-    QOperand Vc{M + r0 + (size_t)p * ld, M + r0 + (size_t)p * ld, ld, ld, INT_MAX, true, -1};
-    const int nz = splitk_count(mr, w.nsplit, nullptr);
-    if ((size_t)nz * wdt * n > w.ypart_slots) return ZQI_ERR_ARG;
-    int rc;
-    {
-      ProfScope ps(PROF_BACKTR, s, 32.0 * (double)mr * wdt * n);
-      rc = qgemm_cn_splitk(wdt, n, mr, Vc, X + r0, ldx, w.Ypart, w.nsplit, s);
-    }
-    if (rc) return rc;
-    const int total = wdt * n;
-    hipLaunchKernelGGL(k_sum_parts, dim3(cdiv(total, 256)), dim3(256), 0, s, total, nz, w.Ypart, w.Y);
-    const quat* T = Tf + (size_t)panel * nb * nb;
-    QOperand To{T, T, nb, nb, INT_MAX, false, INT_MIN / 2};
-    QOperand Yo{w.Y, w.Y, wdt, wdt, INT_MAX, false, INT_MIN / 2};
-    rc = qgemm(wdt, n, wdt, 1.0, To, Yo, 0.0, w.Y2, wdt, false, s);
-    if (rc) return rc;
-    QOperand Vn{M + r0 + (size_t)p * ld, M + r0 + (size_t)p * ld, ld, ld, INT_MAX, false, -1};
-    QOperand Y2o{w.Y2, w.Y2, wdt, wdt, INT_MAX, false, INT_MIN / 2};
-    {
-      ProfScope ps(PROF_BACKTR, s, 32.0 * (double)mr * wdt * n);
-      rc = qgemm(mr, n, wdt, -1.0, Vn, Y2o, 1.0, X + r0, ldx, false, s);
-    }
-    if (rc) return rc;
-  }
-  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+size_t backtr_vt_slots(int n, int wbt) {
+  size_t t = 0;
+  for (int p = 0; p < n - 1; p += wbt) t += (size_t)(n - p - 1) * wbt;
+  return t + 1;
 }
 
-size_t backtransform_batched_tasks(int nprob, const int* nh, int nb, int nsplit) {
+size_t backtransform_batched_tasks(int nprob, const int* nh, int wbt, int nsplit) {
   if (nprob <= 0 || nh[0] < 2) return 1;
-  return (size_t)nprob * cdiv(nh[0] - 1, nb) * (std::max(nsplit, 1) + 2) + 1;
+  // per block: G and Y split products (<= nsplit each), VT and the X update
+  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * (2 * std::max(nsplit, 1) + 2) + 1;
 }
 
-int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_h, const int* zrow_d, int nb,
-                          const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz, quat* X,
-                          int ldx, const BacktrWork& w, QGemmTask* tasks, SumTask* sums, Staging& st, hipStream_t s) {
-  (void)zrow_h;
+size_t backtransform_batched_sums(int nprob, const int* nh, int wbt) {
+  if (nprob <= 0 || nh[0] < 2) return 1;
+  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * 2 + 1;
+}
+
+int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, const quat* M, int ld, const quat* e,
+                          const double* Z, int ldz, quat* X, int ldx, const BacktrWork& w, QGemmTask* tasks,
+                          SumTask* sums, Staging& st, hipStream_t s) {
   if (nprob <= 0) return ZQI_OK;
   const int L = nh[0];
+  const int wbt = w.wbt;
   hipLaunchKernelGGL(k_phases, dim3(1, 1, nprob), dim3(PH_T), 0, s, L, e, w.phi, zb);
   hipLaunchKernelGGL(k_phase_apply, dim3(cdiv(L, 256), L, nprob), dim3(256), 0, s, L, w.phi, Z, ldz, X, ldx, zb,
                      zrow_d);
@@ -165,74 +159,123 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_h, co
     using T = std::remove_pointer_t<decltype(p)>;
     return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)z * zb.zs);
   };
-  // per panel (last first): the split products Y = V^* X, their sums, Y2 = T Y, X -= V Y2,
-  // each problem's tasks exactly those backtransform() launches for it
-  const int npan = cdiv(L - 1, nb);
-  struct Step { int y0, ny, s0, ns, t0, nt, x0, nx, maxm_y, maxn, maxtot, maxmr; };
+  const int nblk = cdiv(L - 1, wbt);
+  // block b of problem z: reflectors p = b wbt .. p + wdt - 1 acting on rows r0 = p + 1 .. n - 1
+  struct Blk { int p, wdt, r0, mr; size_t vt; };
+  auto block = [&](int z, int b) {
+    const int n = nh[z];
+    Blk k;
+    k.p = b * wbt;
+    k.wdt = std::min(wbt, n - 1 - k.p);
+    k.r0 = k.p + 1;
+    k.mr = n - k.r0;
+    size_t v = 0;
+    for (int q = 0; q < b; ++q) v += (size_t)(n - q * wbt - 1) * wbt;
+    k.vt = v;
+    return k;
+  };
   std::vector<QGemmTask> tt;
   std::vector<SumTask> ss;
-  std::vector<Step> steps;
-  for (int panel = npan - 1; panel >= 0; --panel) {
-    const int p = panel * nb, r0 = p + 1;
-    Step st_{};
-    st_.y0 = (int)tt.size();
-    st_.s0 = (int)ss.size();
-    std::vector<QGemmTask> ty, tx;
-    for (int z = 0; z < nprob && nh[z] - 1 > p; ++z) {
-      const int n = nh[z];
-      const int wdt = std::min(nb, n - 1 - p);
-      const int mr = n - r0;
-      const quat* Mz = at(M, z);
-      quat* Xz = at(X, z);
-      quat* Yp = at(w.Ypart, z);
-      quat* Yz = at(w.Y, z);
-      quat* Y2z = at(w.Y2, z);
-      const quat* Tz = at(Tf, z) + (size_t)panel * nb * nb;
-      QOperand Vc{Mz + r0 + (size_t)p * ld, Mz + r0 + (size_t)p * ld, ld, ld, INT_MAX, true, -1};
-      QOperand Bo{Xz + r0, Xz + r0, ldx, ldx, INT_MAX, false, INT_MIN / 2};
+  // ---- phase A (every block of every problem): G = V^* V, T, VT = V T^*
+  const int gt0 = 0;
+  int gmaxk = 0;
+  for (int z = 0; z < nprob; ++z)
+    for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
+      const Blk k = block(z, b);
+      const quat* V = at(M, z) + k.r0 + (size_t)k.p * ld;
+      QOperand Vc{V, V, ld, ld, INT_MAX, true, -1};
+      QOperand Vn{V, V, ld, ld, INT_MAX, false, -1};
+      quat* part = at(w.part, z) + (size_t)b * w.nsplit * wbt * wbt;
       int kchunk = 0;
-      const int nz = splitk_count(mr, w.nsplit, &kchunk);
-      if ((size_t)nz * wdt * n > w.ypart_slots) return ZQI_ERR_ARG;
+      const int nz = splitk_count(k.mr, w.nsplit, &kchunk);
+      if ((size_t)(b + 1) * w.nsplit * wbt * wbt > w.part_slots) return ZQI_ERR_ARG;
       for (int q = 0; q < nz; ++q) {
-        const int kb = q * kchunk, ke = std::min(mr, kb + kchunk);
-        tt.push_back(QGemmTask{wdt, n, mr, kb, ke, wdt, 0, 1.0, 0.0, Vc, Bo, Yp + (size_t)q * wdt * n});
+        const int kb = q * kchunk, ke = std::min(k.mr, kb + kchunk);
+        tt.push_back(QGemmTask{k.wdt, k.wdt, k.mr, kb, ke, wbt, 0, 1.0, 0.0, Vc, Vn, part + (size_t)q * wbt * wbt});
       }
-      ss.push_back(SumTask{Yp, Yz, wdt * n, nz});
-      QOperand To{Tz, Tz, nb, nb, INT_MAX, false, INT_MIN / 2};
-      QOperand Yo{Yz, Yz, wdt, wdt, INT_MAX, false, INT_MIN / 2};
-      ty.push_back(QGemmTask{wdt, n, wdt, 0, wdt, wdt, 0, 1.0, 0.0, To, Yo, Y2z});
-      QOperand Vn{Mz + r0 + (size_t)p * ld, Mz + r0 + (size_t)p * ld, ld, ld, INT_MAX, false, -1};
-      QOperand Y2o{Y2z, Y2z, wdt, wdt, INT_MAX, false, INT_MIN / 2};
-      tx.push_back(QGemmTask{mr, n, wdt, 0, wdt, ldx, 0, -1.0, 1.0, Vn, Y2o, Xz + r0});
-      st_.maxm_y = std::max(st_.maxm_y, wdt);
-      st_.maxn = std::max(st_.maxn, n);
-      st_.maxtot = std::max(st_.maxtot, wdt * n);
-      st_.maxmr = std::max(st_.maxmr, mr);
+      ss.push_back(SumTask{part, at(w.G, z) + (size_t)b * wbt * wbt, wbt * wbt, nz});
+      gmaxk = std::max(gmaxk, k.wdt);
     }
-    st_.ny = (int)tt.size() - st_.y0;
-    st_.ns = (int)ss.size() - st_.s0;
-    st_.t0 = (int)tt.size();
-    tt.insert(tt.end(), ty.begin(), ty.end());
-    st_.nt = (int)ty.size();
-    st_.x0 = (int)tt.size();
+  const int ngt = (int)tt.size() - gt0;
+  const int ngs = (int)ss.size();
+  const int vt0 = (int)tt.size();
+  int vtmaxm = 0;
+  for (int z = 0; z < nprob; ++z)
+    for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
+      const Blk k = block(z, b);
+      if (k.vt + (size_t)k.mr * wbt > w.vt_slots) return ZQI_ERR_ARG;
+      const quat* V = at(M, z) + k.r0 + (size_t)k.p * ld;
+      const quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
+      QOperand Vn{V, V, ld, ld, INT_MAX, false, -1};
+      QOperand Tc{T, T, wbt, wbt, INT_MAX, true, INT_MIN / 2};
+      tt.push_back(QGemmTask{k.mr, k.wdt, k.wdt, 0, k.wdt, k.mr, 0, 1.0, 0.0, Vn, Tc, at(w.VT, z) + k.vt});
+      vtmaxm = std::max(vtmaxm, k.mr);
+    }
+  const int nvt = (int)tt.size() - vt0;
+  // ---- phase B, last block first: Y = VT^* X[r0:, :] (split), sums, X[r0:, :] -= V Y
+  struct Step { int y0, ny, s0, ns, x0, nx, maxw, maxn, maxtot, maxmr; double fy, fx; };
+  std::vector<Step> steps;
+  for (int b = nblk - 1; b >= 0; --b) {
+    Step q{};
+    q.y0 = (int)tt.size();
+    q.s0 = (int)ss.size();
+    std::vector<QGemmTask> tx;
+    for (int z = 0; z < nprob && b * wbt < nh[z] - 1; ++z) {
+      const int n = nh[z];
+      const Blk k = block(z, b);
+      const quat* V = at(M, z) + k.r0 + (size_t)k.p * ld;
+      const quat* VT = at(w.VT, z) + k.vt;
+      quat* Xz = at(X, z);
+      quat* part = at(w.part, z);
+      quat* Y = at(w.Y, z);
+      QOperand VTc{VT, VT, k.mr, k.mr, INT_MAX, true, INT_MIN / 2};
+      QOperand Xo{Xz + k.r0, Xz + k.r0, ldx, ldx, INT_MAX, false, INT_MIN / 2};
+      int kchunk = 0;
+      const int nz = splitk_count(k.mr, w.nsplit, &kchunk);
+      if ((size_t)nz * k.wdt * n > w.part_slots) return ZQI_ERR_ARG;
+      for (int c = 0; c < nz; ++c) {
+        const int kb = c * kchunk, ke = std::min(k.mr, kb + kchunk);
+        tt.push_back(QGemmTask{k.wdt, n, k.mr, kb, ke, k.wdt, 0, 1.0, 0.0, VTc, Xo, part + (size_t)c * k.wdt * n});
+      }
+      ss.push_back(SumTask{part, Y, k.wdt * n, nz});
+      QOperand Vn{V, V, ld, ld, INT_MAX, false, -1};
+      QOperand Yo{Y, Y, k.wdt, k.wdt, INT_MAX, false, INT_MIN / 2};
+      tx.push_back(QGemmTask{k.mr, n, k.wdt, 0, k.wdt, ldx, 0, -1.0, 1.0, Vn, Yo, Xz + k.r0});
+      q.maxw = std::max(q.maxw, k.wdt);
+      q.maxn = std::max(q.maxn, n);
+      q.maxtot = std::max(q.maxtot, k.wdt * n);
+      q.maxmr = std::max(q.maxmr, k.mr);
+      q.fy += 32.0 * k.wdt * (double)n * k.mr;
+      q.fx += 32.0 * k.mr * (double)n * k.wdt;
+    }
+    q.ny = (int)tt.size() - q.y0;
+    q.ns = (int)ss.size() - q.s0;
+    q.x0 = (int)tt.size();
     tt.insert(tt.end(), tx.begin(), tx.end());
-    st_.nx = (int)tx.size();
-    steps.push_back(st_);
+    q.nx = (int)tx.size();
+    steps.push_back(q);
   }
-  if (tt.size() > backtransform_batched_tasks(nprob, nh, nb, w.nsplit)) return ZQI_ERR_ARG;
+  if (tt.size() > backtransform_batched_tasks(nprob, nh, wbt, w.nsplit) ||
+      ss.size() > backtransform_batched_sums(nprob, nh, wbt))
+    return ZQI_ERR_ARG;
   int rc = stage_upload(st, tt.data(), tt.size() * sizeof(QGemmTask), tasks, s);
   if (!rc) rc = stage_upload(st, ss.data(), ss.size() * sizeof(SumTask), sums, s);
   if (rc) return rc;
+  // phase A launches
+  if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, s))) return rc;
+  hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(wbt * wbt, 256), 1, ngs), dim3(256), 0, s, sums);
+  hipLaunchKernelGGL(k_tfactor, dim3(nblk, 1, nprob), dim3(256), 0, s, L, wbt, w.G, w.T, zb);
+  if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, s))) return rc;
+  // phase B launches
   for (const Step& q : steps) {
     {
-      ProfScope ps(PROF_BACKTR, s, 0.0);
-      if ((rc = qgemm_tasks(tasks + q.y0, q.ny, q.maxm_y, q.maxn, s))) return rc;
+      ProfScope ps(PROF_BACKTR, s, q.fy);
+      if ((rc = qgemm_tasks(tasks + q.y0, q.ny, q.maxw, q.maxn, true, false, s))) return rc;
     }
     hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(q.maxtot, 256), 1, q.ns), dim3(256), 0, s, sums + q.s0);
-    if ((rc = qgemm_tasks(tasks + q.t0, q.nt, q.maxm_y, q.maxn, s))) return rc;
     {
-      ProfScope ps(PROF_BACKTR, s, 0.0);
-      if ((rc = qgemm_tasks(tasks + q.x0, q.nx, q.maxmr, q.maxn, s))) return rc;
+      ProfScope ps(PROF_BACKTR, s, q.fx);
+      if ((rc = qgemm_tasks(tasks + q.x0, q.nx, q.maxmr, q.maxn, false, false, s))) return rc;
     }
   }
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;

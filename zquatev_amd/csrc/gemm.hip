@@ -24,111 +24,131 @@
 // (zquatev.cc:86-90).
 #include "zq_internal.h"
 #include <climits>
+#include <cstdlib>
 
 namespace zq {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int QBM = 32;   // quaternion rows per workgroup tile
-constexpr int QBN = 64;   // quaternion cols per workgroup tile
-constexpr int QBK = 8;    // quaternion k per LDS stage
+// ---------------------------------------------------------------------------
+// Tiles of 64 x 64 quaternions per workgroup (2 x 2 waves, 32 x 32
+// quaternions = 16 MFMA accumulators per wave), K staged 8 quaternions at a
+// time through double-buffered LDS.  The next stage's operands are fetched
+// into registers before the current stage's MFMAs, so global latency hides
+// behind the wave's own work, and one barrier per stage suffices (a buffer is
+// rewritten only after the barrier that follows its last read).  The
+// conjugate-transpose flags are template parameters (branch-free address
+// math); the sign of the L(A) expansion is applied by flipping the sign bit.
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 8;
 
-__device__ __forceinline__ quat load_opA(const QOperand& A, int i, int k, int m, int K) {
-  // logical A[i][k]; storage either (i, k) or (k, i) conj
+template <bool CT>
+__device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, int K) {
   if (i >= m || k >= K) return qzero();
-  const quat* p;
-  int ldp, kk = k;
-  if (k < A.ksplit) { p = A.p0; ldp = A.ld0; } else { p = A.p1; ldp = A.ld1; kk = k - A.ksplit; }
-  int sr, sc;
-  if (!A.conj_trans) { sr = i; sc = kk; } else { sr = kk; sc = i; }
+  const quat* p = A.p0;
+  int ldp = A.ld0, kk = k;
+  if (k >= A.ksplit) { p = A.p1; ldp = A.ld1; kk = k - A.ksplit; }
+  const int sr = CT ? kk : i, sc = CT ? i : kk;
   if (!(sr > sc + A.mask_off)) return qzero();
-  quat v = qload(p + sr + (size_t)sc * ldp);
-  return A.conj_trans ? qconj(v) : v;
+  const quat v = qload(p + sr + (size_t)sc * ldp);
+  return CT ? qconj(v) : v;
 }
 
-__device__ __forceinline__ quat load_opB(const QOperand& B, int k, int jcol, int K, int nn) {
-  if (k >= K || jcol >= nn) return qzero();
-  const quat* p;
-  int ldp, kk = k;
-  if (k < B.ksplit) { p = B.p0; ldp = B.ld0; } else { p = B.p1; ldp = B.ld1; kk = k - B.ksplit; }
-  int sr, sc;
-  if (!B.conj_trans) { sr = kk; sc = jcol; } else { sr = jcol; sc = kk; }
+template <bool CT>
+__device__ __forceinline__ quat ld_opB(const QOperand& B, int k, int j, int K, int nn) {
+  if (k >= K || j >= nn) return qzero();
+  const quat* p = B.p0;
+  int ldp = B.ld0, kk = k;
+  if (k >= B.ksplit) { p = B.p1; ldp = B.ld1; kk = k - B.ksplit; }
+  const int sr = CT ? j : kk, sc = CT ? kk : j;
   if (!(sr > sc + B.mask_off)) return qzero();
-  quat v = qload(p + sr + (size_t)sc * ldp);
-  return B.conj_trans ? qconj(v) : v;
+  const quat v = qload(p + sr + (size_t)sc * ldp);
+  return CT ? qconj(v) : v;
 }
 
-// One tile (QBM x QBN quaternions at row0, col0) of C = beta C + alpha A B
-// over logical k in [kbeg, kend).
-__device__ __forceinline__ void qgemm_tile(const QGemmTask& t, int row0, int col0, int kbeg, int kend, quat* C) {
+__device__ __forceinline__ double flip_sign(double v, unsigned mask) {
+  return mkd(lo32(v), hi32(v) ^ mask);
+}
+
+template <bool ACT, bool BCT>
+__device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int col0, int kbeg, int kend, quat* C) {
   const int m = t.m, nn = t.nn, K = t.K;
-  const QOperand& A = t.A;
-  const QOperand& B = t.B;
-  __shared__ double As[QBK][QBM][4];
-  __shared__ double Bs[QBK][QBN][4];
+  __shared__ double As[2][GBK][GBM][4];
+  __shared__ double Bs[2][GBK][GBN][4];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1;   // wave tile: rows wr*16..+16, cols wc*32..+32
-
-  // per-lane fragment constants: A-operand real row (lane & 15) is component
-  // cr = (lane & 15) >> 2 of quaternion row aq = lane & 3; real k = lane >> 4
+  const int wr = wv >> 1, wc = wv & 1;   // wave tile: rows wr*32..+32, cols wc*32..+32 (quaternions)
   const int cr = (lane & 15) >> 2, ck = lane >> 4;
   const int acomp = cr ^ ck;
-  // sign[cr][ck]: rows [+,-,-,-],[+,+,+,-],[+,-,+,+],[+,+,-,+]; nibble cr of
-  // 0x428E has bit ck set where the sign is negative
-  const double asign = ((0x428E >> (cr * 4 + ck)) & 1) ? -1.0 : 1.0;
-  const int aq = lane & 3;           // quaternion row within the 4-row MFMA tile
-  const int bkc = lane >> 4;         // component of B
-  const int bj = lane & 15;
-
-  d4 acc[4][2];
+  const unsigned smask = ((0x428E >> (cr * 4 + ck)) & 1) ? 0x80000000u : 0u;
+  const int aq = lane & 3;
+  const int bkc = lane >> 4, bj = lane & 15;
+  // this thread's two A and two B stage elements
+  int ai[2], ak[2], bk[2], bjj[2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = kbeg; k0 < kend; k0 += QBK) {
-    // A tile: QBM x QBK = 256 quaternions, one per thread
-    {
-      int i, k;
-      if (!A.conj_trans) { i = tid & (QBM - 1); k = tid >> 5; }
-      else { k = tid & (QBK - 1); i = tid >> 3; }
-      const int kg = k0 + k;
-      quat v = (kg < kend) ? load_opA(A, row0 + i, kg, m, K) : qzero();
-      As[k][i][0] = v.ar; As[k][i][1] = v.ai; As[k][i][2] = v.br; As[k][i][3] = v.bi;
-    }
-    // B tile: QBK x QBN = 512 quaternions, two per thread
+  for (int h = 0; h < 2; ++h) {
+    if (!ACT) { ai[h] = tid & 63; ak[h] = (tid >> 6) + 4 * h; }
+    else { ak[h] = tid & 7; ai[h] = (tid >> 3) + 32 * h; }
+    if (!BCT) { bk[h] = tid & 7; bjj[h] = (tid >> 3) + 32 * h; }
+    else { bjj[h] = tid & 63; bk[h] = (tid >> 6) + 4 * h; }
+  }
+  quat ra[2], rb[2];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      int k, jj;
-      if (!B.conj_trans) { k = tid & (QBK - 1); jj = (tid >> 3) + 32 * h; }
-      else { jj = tid & (QBN - 1); k = (tid >> 6) + 4 * h; }
-      const int kg = k0 + k;
-      quat v = (kg < kend) ? load_opB(B, kg, col0 + jj, K, nn) : qzero();
-      Bs[k][jj][0] = v.ar; Bs[k][jj][1] = v.ai; Bs[k][jj][2] = v.br; Bs[k][jj][3] = v.bi;
+      const int ka = k0 + ak[h], kb = k0 + bk[h];
+      ra[h] = (ka < kend) ? ld_opA<ACT>(t.A, row0 + ai[h], ka, m, K) : qzero();
+      rb[h] = (kb < kend) ? ld_opB<BCT>(t.B, kb, col0 + bjj[h], K, nn) : qzero();
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
 #pragma unroll
-    for (int kk = 0; kk < QBK; ++kk) {
-      double af[4], bf[2];
+    for (int h = 0; h < 2; ++h) {
+      double2* pa = reinterpret_cast<double2*>(&As[buf][ak[h]][ai[h]][0]);
+      pa[0] = make_double2(ra[h].ar, ra[h].ai);
+      pa[1] = make_double2(ra[h].br, ra[h].bi);
+      double2* pb = reinterpret_cast<double2*>(&Bs[buf][bk[h]][bjj[h]][0]);
+      pb[0] = make_double2(rb[h].ar, rb[h].ai);
+      pb[1] = make_double2(rb[h].br, rb[h].bi);
+    }
+  };
+  d4 acc[8][2];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) af[a] = asign * As[kk][wr * 16 + 4 * a + aq][acomp];
+  for (int a = 0; a < 8; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[kk][wc * 32 + 16 * b + bj][bkc];
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  int cur = 0;
+  if (kbeg < kend) {
+    fetch(kbeg);
+    stash(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+    const bool more = k0 + GBK < kend;
+    if (more) fetch(k0 + GBK);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int kk = 0; kk < GBK; ++kk) {
+      double af[8], bf[2];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) af[a] = flip_sign(As[cur][kk][wr * 32 + 4 * a + aq][acomp], smask);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bf[b] = Bs[cur][kk][wc * 32 + 16 * b + bj][bkc];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
+    if (more) stash(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
-  // epilogue: lane holds quaternion (row wr*16 + 4a + lane/16, col wc*32 + 16b + lane%16)
+  // epilogue: lane holds quaternion (row wr*32 + 4a + lane/16, col wc*32 + 16b + lane%16)
   const double alpha = t.alpha, beta = t.beta;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
+  for (int a = 0; a < 8; ++a) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int i = row0 + wr * 16 + 4 * a + (lane >> 4);
+      const int i = row0 + wr * 32 + 4 * a + (lane >> 4);
       const int jj = col0 + wc * 32 + 16 * b + (lane & 15);
       if (i < m && jj < nn && (!t.lower_only || i >= jj)) {
         quat* cp = C + i + (size_t)jj * t.ldc;
@@ -140,12 +160,11 @@ __device__ __forceinline__ void qgemm_tile(const QGemmTask& t, int row0, int col
   }
 }
 
-// One task per launch.  gridDim.z > 1 splits [t.kbeg, t.kend) into chunks of
-// kchunk; chunk z writes C + z * cstride (the task then has beta = 0).
-__global__ void __launch_bounds__(256) k_qgemm(QGemmTask t, int kchunk, size_t cstride) {
-  const int row0 = blockIdx.x * QBM;
-  const int col0 = blockIdx.y * QBN;
-  if (t.lower_only && row0 + QBM - 1 < col0) return;
+template <bool ACT, bool BCT>
+__global__ void __launch_bounds__(256, 2) k_qgemm2(QGemmTask t, int kchunk, size_t cstride) {
+  const int row0 = blockIdx.x * GBM;
+  const int col0 = blockIdx.y * GBN;
+  if (t.lower_only && row0 + GBM - 1 < col0) return;
   int kbeg = t.kbeg, kend = t.kend;
   quat* C = t.C;
   if (gridDim.z > 1) {
@@ -153,41 +172,53 @@ __global__ void __launch_bounds__(256) k_qgemm(QGemmTask t, int kchunk, size_t c
     kend = min(t.kend, kbeg + kchunk);
     C += (size_t)blockIdx.z * cstride;
   }
-  qgemm_tile(t, row0, col0, kbeg, kend, C);
+  qgemm2_tile<ACT, BCT>(t, row0, col0, kbeg, kend, C);
 }
 
-// Many independent tasks per launch (a batch of problems, or the K chunks of
-// a split product listed as tasks): task blockIdx.z.
-__global__ void __launch_bounds__(256) k_qgemm_batch(const QGemmTask* __restrict__ tasks) {
+template <bool ACT, bool BCT>
+__global__ void __launch_bounds__(256, 2) k_qgemm2_batch(const QGemmTask* __restrict__ tasks) {
   const QGemmTask t = tasks[blockIdx.z];
-  const int row0 = blockIdx.x * QBM;
-  const int col0 = blockIdx.y * QBN;
+  const int row0 = blockIdx.x * GBM;
+  const int col0 = blockIdx.y * GBN;
   if (row0 >= t.m || col0 >= t.nn) return;
-  if (t.lower_only && row0 + QBM - 1 < col0) return;
-  qgemm_tile(t, row0, col0, t.kbeg, t.kend, t.C);
+  if (t.lower_only && row0 + GBM - 1 < col0) return;
+  qgemm2_tile<ACT, BCT>(t, row0, col0, t.kbeg, t.kend, t.C);
+}
+
+static void launch_gemm2(bool act, bool bct, dim3 grid, hipStream_t s, const QGemmTask& t, int kchunk,
+                         size_t cstride) {
+  if (!act && !bct) launch_k(k_qgemm2<false, false>, grid, dim3(256), 0, s, t, kchunk, cstride);
+  else if (act && !bct) launch_k(k_qgemm2<true, false>, grid, dim3(256), 0, s, t, kchunk, cstride);
+  else if (!act && bct) launch_k(k_qgemm2<false, true>, grid, dim3(256), 0, s, t, kchunk, cstride);
+  else launch_k(k_qgemm2<true, true>, grid, dim3(256), 0, s, t, kchunk, cstride);
+}
+
+static void launch_gemm2_batch(bool act, bool bct, dim3 grid, hipStream_t s, const QGemmTask* tasks) {
+  if (!act && !bct) launch_k(k_qgemm2_batch<false, false>, grid, dim3(256), 0, s, tasks);
+  else if (act && !bct) launch_k(k_qgemm2_batch<true, false>, grid, dim3(256), 0, s, tasks);
+  else if (!act && bct) launch_k(k_qgemm2_batch<false, true>, grid, dim3(256), 0, s, tasks);
+  else launch_k(k_qgemm2_batch<true, true>, grid, dim3(256), 0, s, tasks);
 }
 
 int qgemm(int m, int nn, int k, double alpha, const QOperand& A, const QOperand& B, double beta, quat* C, int ldc,
           bool lower_only, hipStream_t s) {
   if (m <= 0 || nn <= 0) return ZQI_OK;
-  dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, 1);
   QGemmTask t{m, nn, k, 0, k, ldc, lower_only ? 1 : 0, alpha, beta, A, B, C};
-  launch_k(k_qgemm, grid, dim3(256), 0, s, t, k, (size_t)0);
+  launch_gemm2(A.conj_trans, B.conj_trans, dim3((m + GBM - 1) / GBM, (nn + GBN - 1) / GBN, 1), s, t, k, (size_t)0);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
-int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, hipStream_t s) {
+int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool act, bool bct, hipStream_t s) {
   if (ntasks <= 0 || maxm <= 0 || maxnn <= 0) return ZQI_OK;
-  dim3 grid((maxm + QBM - 1) / QBM, (maxnn + QBN - 1) / QBN, ntasks);
-  launch_k(k_qgemm_batch, grid, dim3(256), 0, s, tasks_dev);
+  launch_gemm2_batch(act, bct, dim3((maxm + GBM - 1) / GBM, (maxnn + GBN - 1) / GBN, ntasks), s, tasks_dev);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
 int splitk_count(int K, int nsplit, int* kchunk_out) {
   if (nsplit < 1) nsplit = 1;
   int kchunk = (K + nsplit - 1) / nsplit;
-  kchunk = ((kchunk + QBK - 1) / QBK) * QBK;
-  if (kchunk < QBK) kchunk = QBK;
+  kchunk = ((kchunk + GBK - 1) / GBK) * GBK;
+  if (kchunk < GBK) kchunk = GBK;
   if (kchunk_out) *kchunk_out = kchunk;
   const int nz = (K + kchunk - 1) / kchunk;
   return nz > 0 ? nz : 1;
@@ -199,9 +230,9 @@ int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int 
   int kchunk = 0;
   const int nz = splitk_count(K, nsplit, &kchunk);
   QOperand Bo{B, B, ldb, ldb, INT_MAX, false, INT_MIN / 2};
-  dim3 grid((m + QBM - 1) / QBM, (nn + QBN - 1) / QBN, nz);
   QGemmTask t{m, nn, K, 0, K, m, 0, 1.0, 0.0, A, Bo, Ypart};
-  launch_k(k_qgemm, grid, dim3(256), 0, s, t, kchunk, (size_t)m * nn);
+  launch_gemm2(A.conj_trans, false, dim3((m + GBM - 1) / GBM, (nn + GBN - 1) / GBN, nz), s, t, kchunk,
+               (size_t)m * nn);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

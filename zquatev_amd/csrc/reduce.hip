@@ -340,19 +340,18 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
 
 // p = y - V (W^* u) - W (V^* u) with y = M u = gamma (M x) + M[:, s0] (u1 - gamma x1),
 // (M x) from the matvec partials; stored in W[:, j].  Also stores u into
-// M[c+1:, c], partial u^* p into KP, and (block 0) column j of the strict
-// upper part of G = V^* V (the T factor is formed from G after the panel).
+// M[c+1:, c] and partial u^* p into KP.
 template <int KPT>
 __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                                 quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                                 const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
                                                 int nb, const double* __restrict__ scal,
-                                                double* __restrict__ KP, quat* __restrict__ Gm, ZB zb) {
+                                                double* __restrict__ KP, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     if (c + 1 + (int)blockIdx.x * RQ >= n) return;
     M = zoff(M, zb); W = zoff(W, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb);
-    scal = zoff(scal, zb); KP = zoff(KP, zb); Gm = zoff(Gm, zb);
+    scal = zoff(scal, zb); KP = zoff(KP, zb);
   }
   __shared__ quat a[NBMAX], b[NBMAX];
   __shared__ quat red[16][RQ];
@@ -390,9 +389,6 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
     b[tid] = qload(TS + nb + tid);
   }
   __syncthreads();
-  if (blockIdx.x == 0) {
-    for (int rr = tid; rr < nb; rr += RB) qstore(Gm + rr + (size_t)j * nb, rr < j ? b[rr] : qzero());
-  }
   quat acc = qscale(yi, gam);   // this thread's share of y(i) minus its share of the corrections
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
@@ -445,37 +441,6 @@ __global__ void __launch_bounds__(RB) k_wfinal(const quat* __restrict__ M, int l
   }
 }
 
-// T = (I + triu(G, 1))^{-1} for each panel: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
-// T[j, j] = 1, zero below (equivalent to the usual compact-WY recurrence
-// since u^* u = 2 makes every tau 1).  One workgroup per panel.
-__global__ void __launch_bounds__(RB) k_tfactor(int n, int nb, const quat* __restrict__ Gall, quat* __restrict__ Tall,
-                                                ZB zb) {
-  const int panel = blockIdx.x;
-  if (zb.nvec) {
-    n = zb.nvec[blockIdx.z];
-    if (panel >= (n - 1 + nb - 1) / nb) return;
-    Gall = zoff(Gall, zb); Tall = zoff(Tall, zb);
-  }
-  const int p = panel * nb;
-  const int wdt = min(nb, n - 1 - p);
-  const quat* G = Gall + (size_t)panel * nb * nb;
-  quat* T = Tall + (size_t)panel * nb * nb;
-  for (int jj = 0; jj < nb; ++jj) {
-    for (int r = threadIdx.x; r < nb; r += RB) {
-      quat acc = qzero();
-      if (jj < wdt && r < jj) {
-#pragma unroll 4
-        for (int l = r; l < jj; ++l) qfma(acc, qload(T + r + (size_t)l * nb), qload(G + l + (size_t)jj * nb));
-        acc = qscale(acc, -1.0);
-      } else if (r == jj) {
-        acc = quat{1.0, 0.0, 0.0, 0.0};
-      }
-      qstore(T + r + (size_t)jj * nb, acc);
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* __restrict__ d, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
@@ -524,7 +489,6 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
   int panel = 0;
   for (int p = 0; p < n - 1; p += nb, ++panel) {
     const int wdt = (nb < n - 1 - p) ? nb : (n - 1 - p);
-    quat* Gp = w.Gm + (size_t)panel * nb * nb;
     for (int j = 0; j < wdt; ++j) {
       const int c = p + j;
       const int nrow = n - c;                      // rows c..n-1
@@ -542,7 +506,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
                  w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, ZB{nullptr, 0});
       }
       const int nb2 = cdiv(sz, RQ);
-      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal, w.KP, Gp,
+      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal, w.KP,
                      ZB{nullptr, 0});
       if (j == wdt - 1)
         hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2,
@@ -561,7 +525,6 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
     }
   }
   hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
-  hipLaunchKernelGGL(k_tfactor, dim3(panel), dim3(RB), 0, s, n, nb, w.Gm, w.Tf, ZB{nullptr, 0});
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -618,9 +581,8 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
                        w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, zb);
     const int nb2 = cdiv(sz, RQ);
-    quat* Gp = w.Gm + (size_t)(c / nb) * nb * nb;
     launch_wreduce(kpt, dim3(nb2, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb, w.scal,
-                   w.KP, Gp, zb);
+                   w.KP, zb);
     // w of the last column of a panel: every active problem at a panel end,
     // else those whose last column is c
     bool fin = (j == nb - 1);
@@ -631,11 +593,10 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     if (j == nb - 1 || c == L - 2) {
       const int P = c / nb;
       const int cnt = toff[P + 1] - toff[P];
-      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], s))) return rc;
+      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, s))) return rc;
     }
   }
   hipLaunchKernelGGL(k_last_diag, dim3(1, 1, nprob), dim3(64), 0, s, M, ld, L, d, zb);
-  if (npan > 0) hipLaunchKernelGGL(k_tfactor, dim3(npan, 1, nprob), dim3(RB), 0, s, L, nb, w.Gm, w.Tf, zb);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

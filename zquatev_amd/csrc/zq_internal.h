@@ -246,8 +246,8 @@ __device__ __forceinline__ T* zoff(T* p, const ZB& zb) {
 }
 
 // reduce.hip: quaternion Householder tridiagonalisation of the lower triangle
-// of M (n x n, ld).  Writes d (n), e (n-1 quaternions: the subdiagonal),
-// Tf (per panel nb x nb compact-WY factors) and leaves u_c in M[c+1:, c].
+// of M (n x n, ld).  Writes d (n), e (n-1 quaternions: the subdiagonal) and
+// leaves the reflector u_c in M[c+1:, c].
 struct ReduceWork {
   quat* W;        // n x nb (global row index)
   quat* Prow;     // matvec row partials: ceil(n/32) x n
@@ -258,8 +258,6 @@ struct ReduceWork {
   double* KP;     // ceil(n/16)
   double* part;   // ceil(n/16)
   double* scal;   // 16
-  quat* Tf;       // ceil((n-1)/nb) x nb x nb
-  quat* Gm;       // same shape: strict upper part of V^* V per panel
 };
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s);
 struct QGemmTask;
@@ -318,30 +316,37 @@ int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStrea
 int stage_end(Staging& st, hipStream_t s);
 void stage_free(Staging& st);
 
-// backtr.hip
+// backtr.hip: X = Q Phi Z with the reflectors u_c kept in M, applied in
+// blocks of wbt consecutive reflectors (compact WY, I - V T V^*).
 struct BacktrWork {
   quat* phi;          // n
-  quat* Ypart;        // ypart_slots
-  size_t ypart_slots;
+  quat* part;         // split-K partials: part_slots
+  size_t part_slots;  // >= nsplit * wbt * n
   int nsplit;
-  quat* Y;            // nb x n
-  quat* Y2;           // nb x n
+  int wbt;
+  quat* Y;            // wbt x n
+  quat* G;            // nblk x wbt x wbt: V^* V per block
+  quat* T;            // nblk x wbt x wbt: T = (I + triu(G, 1))^{-1}
+  quat* VT;           // V T^* of every block, block b at vt_offset(b), ld = its row count
+  size_t vt_slots;
 };
-int backtransform(int n, int nb, const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz,
-                  quat* X, int ldx, const BacktrWork& w, hipStream_t s);
-// Batched form (layout as reduce_batched): problem z's real eigenvectors are
-// Z + zrow_h[z] (ld ldz, the forest layout); zrow_d is the device copy.
-// tasks / sums: device room for backtransform_batched_tasks entries each.
+size_t backtr_vt_slots(int n, int wbt);
 struct SumTask {
   const quat* parts;
   quat* out;
   int total;
   int nz;
 };
-size_t backtransform_batched_tasks(int nprob, const int* nh, int nb, int nsplit);
-int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_h, const int* zrow_d, int nb,
-                          const quat* M, int ld, const quat* Tf, const quat* e, const double* Z, int ldz, quat* X,
-                          int ldx, const BacktrWork& w, QGemmTask* tasks, SumTask* sums, Staging& st, hipStream_t s);
+// task-table sizes for a set of problems (sizes nh descending)
+size_t backtransform_batched_tasks(int nprob, const int* nh, int wbt, int nsplit);
+size_t backtransform_batched_sums(int nprob, const int* nh, int wbt);
+// Problems as for reduce_batched (zb; a single solve passes nprob = 1 and
+// ZB{nullptr, 0}); problem z's real eigenvectors are Z + zrow[z] (ld ldz),
+// zrow_d the device copy (null for a single solve).  tasks / sums: device
+// room for the task tables.
+int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, const quat* M, int ld, const quat* e,
+                          const double* Z, int ldz, quat* X, int ldx, const BacktrWork& w, QGemmTask* tasks,
+                          SumTask* sums, Staging& st, hipStream_t s);
 
 // layout.hip
 // One problem of a batch: the caller's D (in and out, ld2 complex), its
@@ -392,8 +397,10 @@ struct QGemmTask {
   quat* C;
 };
 // ntasks independent tasks (device array) in one launch; maxm / maxnn bound
-// their shapes.  Each task computes exactly what qgemm would for it.
-int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, hipStream_t s);
+// their shapes; every task has the given operand conjugation flags.  Each task
+// computes exactly what qgemm would for it.
+int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool a_conj_trans, bool b_conj_trans,
+                hipStream_t s);
 // number of K chunks qgemm_cn_splitk uses for a given request
 int splitk_count(int K, int nsplit, int* kchunk_out);
 // Ypart[z] (m x nn, ld m) = A B over the z-th K chunk (A logical m x K, B K x nn)
