@@ -262,20 +262,20 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
   if (!rc) rc = stage_upload(st, ss.data(), ss.size() * sizeof(SumTask), sums, s);
   if (rc) return rc;
   // phase A launches
-  if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, s))) return rc;
+  if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, false, s))) return rc;
   hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(wbt * wbt, 256), 1, ngs), dim3(256), 0, s, sums);
   hipLaunchKernelGGL(k_tfactor, dim3(nblk, 1, nprob), dim3(256), 0, s, L, wbt, w.G, w.T, zb);
-  if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, s))) return rc;
+  if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, false, s))) return rc;
   // phase B launches
   for (const Step& q : steps) {
     {
       ProfScope ps(PROF_BACKTR, s, q.fy);
-      if ((rc = qgemm_tasks(tasks + q.y0, q.ny, q.maxw, q.maxn, true, false, s))) return rc;
+      if ((rc = qgemm_tasks(tasks + q.y0, q.ny, q.maxw, q.maxn, true, false, false, s))) return rc;
     }
     hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(q.maxtot, 256), 1, q.ns), dim3(256), 0, s, sums + q.s0);
     {
       ProfScope ps(PROF_BACKTR, s, q.fx);
-      if ((rc = qgemm_tasks(tasks + q.x0, q.nx, q.maxmr, q.maxn, false, false, s))) return rc;
+      if ((rc = qgemm_tasks(tasks + q.x0, q.nx, q.maxmr, q.maxn, false, false, false, s))) return rc;
     }
   }
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;

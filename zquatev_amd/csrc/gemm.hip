@@ -31,8 +31,9 @@ namespace zq {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Tiles of 64 x 64 quaternions per workgroup (2 x 2 waves, 32 x 32
-// quaternions = 16 MFMA accumulators per wave), K staged 8 quaternions at a
+// Tiles of TBM x 64 quaternions per workgroup, TBM = 64 or 32 (2 x 2 waves,
+// TBM/2 x 32 quaternions = TBM/4 MFMA accumulators per wave; the smaller tile
+// doubles the workgroups of products too small to fill the device), K staged 8 quaternions at a
 // time through double-buffered LDS.  The next stage's operands are fetched
 // into registers before the current stage's MFMAs, so global latency hides
 // behind the wave's own work, and one barrier per stage suffices (a buffer is
@@ -40,7 +41,12 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // conjugate-transpose flags are template parameters (branch-free address
 // math); the sign of the L(A) expansion is applied by flipping the sign bit.
 // ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 8;
+constexpr int GBN = 64, GBK = 8;
+// LDS layouts (doubles).  A: [k][row][component], B: [k][component][column],
+// each k slice padded by 2 doubles, so that fragment reads (16 consecutive
+// doubles per component group) and the stage writes of either operand
+// orientation (8 consecutive k per 8 threads) fall in distinct banks.
+constexpr int BSK = GBN * 4 + 2;   // B k-slice stride
 
 template <bool CT>
 __device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, int K) {
@@ -70,51 +76,67 @@ __device__ __forceinline__ double flip_sign(double v, unsigned mask) {
   return mkd(lo32(v), hi32(v) ^ mask);
 }
 
-template <bool ACT, bool BCT>
+template <int TBM, bool ACT, bool BCT>
 __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int col0, int kbeg, int kend, quat* C) {
+  constexpr int AH = TBM / 32;          // A stage elements per thread
+  constexpr int AK = TBM * 4 + 2;       // A k-slice stride (doubles)
+  constexpr int WR = TBM / 2;           // quaternion rows per wave
+  constexpr int NA = WR / 4;            // 4-row MFMA tiles per wave
   const int m = t.m, nn = t.nn, K = t.K;
-  __shared__ double As[2][GBK][GBM][4];
-  __shared__ double Bs[2][GBK][GBN][4];
+  __shared__ double As[2][GBK * AK];
+  __shared__ double Bs[2][GBK * BSK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1;   // wave tile: rows wr*32..+32, cols wc*32..+32 (quaternions)
+  const int wr = wv >> 1, wc = wv & 1;   // wave tile: rows wr*WR..+WR, cols wc*32..+32 (quaternions)
   const int cr = (lane & 15) >> 2, ck = lane >> 4;
   const int acomp = cr ^ ck;
   const unsigned smask = ((0x428E >> (cr * 4 + ck)) & 1) ? 0x80000000u : 0u;
   const int aq = lane & 3;
   const int bkc = lane >> 4, bj = lane & 15;
-  // this thread's two A and two B stage elements
-  int ai[2], ak[2], bk[2], bjj[2];
+  // this thread's A and B stage elements
+  int ai[AH], ak[AH], bk[2], bjj[2];
+#pragma unroll
+  for (int h = 0; h < AH; ++h) {
+    if (!ACT) { ai[h] = tid % TBM; ak[h] = tid / TBM + (256 / TBM) * h; }
+    else { ak[h] = tid & 7; ai[h] = (tid >> 3) + 32 * h; }
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    if (!ACT) { ai[h] = tid & 63; ak[h] = (tid >> 6) + 4 * h; }
-    else { ak[h] = tid & 7; ai[h] = (tid >> 3) + 32 * h; }
     if (!BCT) { bk[h] = tid & 7; bjj[h] = (tid >> 3) + 32 * h; }
     else { bjj[h] = tid & 63; bk[h] = (tid >> 6) + 4 * h; }
   }
-  quat ra[2], rb[2];
+  quat ra[AH], rb[2];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int ka = k0 + ak[h], kb = k0 + bk[h];
+    for (int h = 0; h < AH; ++h) {
+      const int ka = k0 + ak[h];
       ra[h] = (ka < kend) ? ld_opA<ACT>(t.A, row0 + ai[h], ka, m, K) : qzero();
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kb = k0 + bk[h];
       rb[h] = (kb < kend) ? ld_opB<BCT>(t.B, kb, col0 + bjj[h], K, nn) : qzero();
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      double2* pa = reinterpret_cast<double2*>(&As[buf][ak[h]][ai[h]][0]);
+    for (int h = 0; h < AH; ++h) {
+      double2* pa = reinterpret_cast<double2*>(&As[buf][ak[h] * AK + ai[h] * 4]);
       pa[0] = make_double2(ra[h].ar, ra[h].ai);
       pa[1] = make_double2(ra[h].br, ra[h].bi);
-      double2* pb = reinterpret_cast<double2*>(&Bs[buf][bk[h]][bjj[h]][0]);
-      pb[0] = make_double2(rb[h].ar, rb[h].ai);
-      pb[1] = make_double2(rb[h].br, rb[h].bi);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double* pb = &Bs[buf][bk[h] * BSK + bjj[h]];
+      pb[0] = rb[h].ar;
+      pb[GBN] = rb[h].ai;
+      pb[2 * GBN] = rb[h].br;
+      pb[3 * GBN] = rb[h].bi;
     }
   };
-  d4 acc[8][2];
+  d4 acc[NA][2];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
   int cur = 0;
@@ -128,13 +150,13 @@ __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int co
     if (more) fetch(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; ++kk) {
-      double af[8], bf[2];
+      double af[NA], bf[2];
 #pragma unroll
-      for (int a = 0; a < 8; ++a) af[a] = flip_sign(As[cur][kk][wr * 32 + 4 * a + aq][acomp], smask);
+      for (int a = 0; a < NA; ++a) af[a] = flip_sign(As[cur][kk * AK + (wr * WR + 4 * a + aq) * 4 + acomp], smask);
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[cur][kk][wc * 32 + 16 * b + bj][bkc];
+      for (int b = 0; b < 2; ++b) bf[b] = Bs[cur][kk * BSK + bkc * GBN + wc * 32 + 16 * b + bj];
 #pragma unroll
-      for (int a = 0; a < 8; ++a)
+      for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
@@ -142,13 +164,13 @@ __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int co
     __syncthreads();
     cur ^= 1;
   }
-  // epilogue: lane holds quaternion (row wr*32 + 4a + lane/16, col wc*32 + 16b + lane%16)
+  // epilogue: lane holds quaternion (row wr*WR + 4a + lane/16, col wc*32 + 16b + lane%16)
   const double alpha = t.alpha, beta = t.beta;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
+  for (int a = 0; a < NA; ++a) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int i = row0 + wr * 32 + 4 * a + (lane >> 4);
+      const int i = row0 + wr * WR + 4 * a + (lane >> 4);
       const int jj = col0 + wc * 32 + 16 * b + (lane & 15);
       if (i < m && jj < nn && (!t.lower_only || i >= jj)) {
         quat* cp = C + i + (size_t)jj * t.ldc;
@@ -160,11 +182,53 @@ __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int co
   }
 }
 
-template <bool ACT, bool BCT>
-__global__ void __launch_bounds__(256, 2) k_qgemm2(QGemmTask t, int kchunk, size_t cstride) {
-  const int row0 = blockIdx.x * GBM;
-  const int col0 = blockIdx.y * GBN;
-  if (t.lower_only && row0 + GBM - 1 < col0) return;
+// Workgroups are dealt to the 8 XCDs round-robin by launch order.  The tile
+// index is remapped so that each XCD works through a contiguous range of
+// tiles (column-major over the tile grid), which share operand tiles in its
+// L2: XCD k = t mod 8 owns floor(T/8) (+1 for k < T mod 8) consecutive ones.
+__device__ __forceinline__ int xcd_remap(int t, int T) {
+  const int k = t & 7;
+  return k * (T >> 3) + min(k, T & 7) + (t >> 3);
+}
+
+// Lower-triangle products enumerate only the tiles that meet the lower
+// triangle: column block y (64 wide) has row tiles x >= (64 y) / TBM.
+__host__ __device__ inline int lower_tiles(int m, int tbm) {
+  const int gx = (m + tbm - 1) / tbm, gy = (m + 63) / 64;
+  int cnt = 0;
+  for (int y = 0; y < gy; ++y) cnt += gx - (64 * y) / tbm;
+  return cnt;
+}
+
+template <int TBM>
+__device__ __forceinline__ bool tile_of(const QGemmTask& t, int u, int gx_rect, int& tx, int& ty) {
+  if (!t.lower_only) {
+    tx = u % gx_rect;
+    ty = u / gx_rect;
+    return true;
+  }
+  const int gx = (t.m + TBM - 1) / TBM, gy = (t.nn + 63) / 64;
+  int acc = 0;
+  for (int y = 0; y < gy; ++y) {
+    const int x0 = (64 * y) / TBM;
+    const int cnt = gx - x0;
+    if (u < acc + cnt) {
+      tx = x0 + (u - acc);
+      ty = y;
+      return true;
+    }
+    acc += cnt;
+  }
+  return false;
+}
+
+// grid: x = tiles (rectangular: gx * gy of them, gx given; lower: lower_tiles), z = K chunks
+template <int TBM, bool ACT, bool BCT>
+__global__ void __launch_bounds__(256, 2) k_qgemm2(QGemmTask t, int kchunk, size_t cstride, int gx) {
+  int tx, ty;
+  if (!tile_of<TBM>(t, xcd_remap(blockIdx.x, gridDim.x), gx, tx, ty)) return;
+  const int row0 = tx * TBM;
+  const int col0 = ty * GBN;
   int kbeg = t.kbeg, kend = t.kend;
   quat* C = t.C;
   if (gridDim.z > 1) {
@@ -172,45 +236,85 @@ __global__ void __launch_bounds__(256, 2) k_qgemm2(QGemmTask t, int kchunk, size
     kend = min(t.kend, kbeg + kchunk);
     C += (size_t)blockIdx.z * cstride;
   }
-  qgemm2_tile<ACT, BCT>(t, row0, col0, kbeg, kend, C);
+  qgemm2_tile<TBM, ACT, BCT>(t, row0, col0, kbeg, kend, C);
 }
 
-template <bool ACT, bool BCT>
+// grid: x = tiles of the largest task, z = tasks
+template <int TBM, bool ACT, bool BCT>
 __global__ void __launch_bounds__(256, 2) k_qgemm2_batch(const QGemmTask* __restrict__ tasks) {
   const QGemmTask t = tasks[blockIdx.z];
-  const int row0 = blockIdx.x * GBM;
-  const int col0 = blockIdx.y * GBN;
-  if (row0 >= t.m || col0 >= t.nn) return;
-  if (t.lower_only && row0 + GBM - 1 < col0) return;
-  qgemm2_tile<ACT, BCT>(t, row0, col0, t.kbeg, t.kend, t.C);
+  // tiles of this task: the launch covers the largest task's; gx from the task
+  const int gx = (t.m + TBM - 1) / TBM;
+  const int ntile = t.lower_only ? lower_tiles(t.m, TBM) : gx * ((t.nn + GBN - 1) / GBN);
+  const int u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= ntile) return;
+  int tx, ty;
+  if (!tile_of<TBM>(t, u, gx, tx, ty)) return;
+  const int row0 = tx * TBM;
+  const int col0 = ty * GBN;
+  qgemm2_tile<TBM, ACT, BCT>(t, row0, col0, t.kbeg, t.kend, t.C);
 }
 
-static void launch_gemm2(bool act, bool bct, dim3 grid, hipStream_t s, const QGemmTask& t, int kchunk,
-                         size_t cstride) {
-  if (!act && !bct) launch_k(k_qgemm2<false, false>, grid, dim3(256), 0, s, t, kchunk, cstride);
-  else if (act && !bct) launch_k(k_qgemm2<true, false>, grid, dim3(256), 0, s, t, kchunk, cstride);
-  else if (!act && bct) launch_k(k_qgemm2<false, true>, grid, dim3(256), 0, s, t, kchunk, cstride);
-  else launch_k(k_qgemm2<true, true>, grid, dim3(256), 0, s, t, kchunk, cstride);
+// Tile rows: 64 when the launch has enough 64 x 64 tiles to fill the device
+// (two per compute unit), else 32 (twice the workgroups for small products).
+static int device_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (!cached[dev]) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
 }
 
-static void launch_gemm2_batch(bool act, bool bct, dim3 grid, hipStream_t s, const QGemmTask* tasks) {
-  if (!act && !bct) launch_k(k_qgemm2_batch<false, false>, grid, dim3(256), 0, s, tasks);
-  else if (act && !bct) launch_k(k_qgemm2_batch<true, false>, grid, dim3(256), 0, s, tasks);
-  else if (!act && bct) launch_k(k_qgemm2_batch<false, true>, grid, dim3(256), 0, s, tasks);
-  else launch_k(k_qgemm2_batch<true, true>, grid, dim3(256), 0, s, tasks);
+static int tile_rows(long tiles64) { return tiles64 >= 2L * device_cus() ? 64 : 32; }
+
+template <int TBM>
+static void launch_gemm2_t(bool act, bool bct, hipStream_t s, const QGemmTask& t, int nz, int kchunk,
+                           size_t cstride) {
+  const int gx = (t.m + TBM - 1) / TBM;
+  const int nt = t.lower_only ? lower_tiles(t.m, TBM) : gx * ((t.nn + GBN - 1) / GBN);
+  const dim3 grid(nt, 1, nz);
+  if (!act && !bct) launch_k(k_qgemm2<TBM, false, false>, grid, dim3(256), 0, s, t, kchunk, cstride, gx);
+  else if (act && !bct) launch_k(k_qgemm2<TBM, true, false>, grid, dim3(256), 0, s, t, kchunk, cstride, gx);
+  else if (!act && bct) launch_k(k_qgemm2<TBM, false, true>, grid, dim3(256), 0, s, t, kchunk, cstride, gx);
+  else launch_k(k_qgemm2<TBM, true, true>, grid, dim3(256), 0, s, t, kchunk, cstride, gx);
+}
+
+static void launch_gemm2(bool act, bool bct, hipStream_t s, const QGemmTask& t, int nz, int kchunk, size_t cstride) {
+  const long t64 = (long)(t.lower_only ? lower_tiles(t.m, 64) : ((t.m + 63) / 64) * ((t.nn + GBN - 1) / GBN)) * nz;
+  if (tile_rows(t64) == 64) launch_gemm2_t<64>(act, bct, s, t, nz, kchunk, cstride);
+  else launch_gemm2_t<32>(act, bct, s, t, nz, kchunk, cstride);
+}
+
+template <int TBM>
+static void launch_gemm2_batch_t(bool act, bool bct, dim3 grid, hipStream_t s, const QGemmTask* tasks) {
+  if (!act && !bct) launch_k(k_qgemm2_batch<TBM, false, false>, grid, dim3(256), 0, s, tasks);
+  else if (act && !bct) launch_k(k_qgemm2_batch<TBM, true, false>, grid, dim3(256), 0, s, tasks);
+  else if (!act && bct) launch_k(k_qgemm2_batch<TBM, false, true>, grid, dim3(256), 0, s, tasks);
+  else launch_k(k_qgemm2_batch<TBM, true, true>, grid, dim3(256), 0, s, tasks);
 }
 
 int qgemm(int m, int nn, int k, double alpha, const QOperand& A, const QOperand& B, double beta, quat* C, int ldc,
           bool lower_only, hipStream_t s) {
   if (m <= 0 || nn <= 0) return ZQI_OK;
   QGemmTask t{m, nn, k, 0, k, ldc, lower_only ? 1 : 0, alpha, beta, A, B, C};
-  launch_gemm2(A.conj_trans, B.conj_trans, dim3((m + GBM - 1) / GBM, (nn + GBN - 1) / GBN, 1), s, t, k, (size_t)0);
+  launch_gemm2(A.conj_trans, B.conj_trans, s, t, 1, k, (size_t)0);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
-int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool act, bool bct, hipStream_t s) {
+int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool act, bool bct, bool lower,
+                hipStream_t s) {
   if (ntasks <= 0 || maxm <= 0 || maxnn <= 0) return ZQI_OK;
-  launch_gemm2_batch(act, bct, dim3((maxm + GBM - 1) / GBM, (maxnn + GBN - 1) / GBN, ntasks), s, tasks_dev);
+  auto ntiles = [&](int tbm) {
+    return lower ? lower_tiles(maxm, tbm) : ((maxm + tbm - 1) / tbm) * ((maxnn + GBN - 1) / GBN);
+  };
+  if (tile_rows((long)ntiles(64) * ntasks) == 64)
+    launch_gemm2_batch_t<64>(act, bct, dim3(ntiles(64), 1, ntasks), s, tasks_dev);
+  else
+    launch_gemm2_batch_t<32>(act, bct, dim3(ntiles(32), 1, ntasks), s, tasks_dev);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -222,18 +326,6 @@ int splitk_count(int K, int nsplit, int* kchunk_out) {
   if (kchunk_out) *kchunk_out = kchunk;
   const int nz = (K + kchunk - 1) / kchunk;
   return nz > 0 ? nz : 1;
-}
-
-int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int ldb, quat* Ypart, int nsplit,
-                    hipStream_t s) {
-  if (m <= 0 || nn <= 0) return ZQI_OK;
-  int kchunk = 0;
-  const int nz = splitk_count(K, nsplit, &kchunk);
-  QOperand Bo{B, B, ldb, ldb, INT_MAX, false, INT_MIN / 2};
-  QGemmTask t{m, nn, K, 0, K, m, 0, 1.0, 0.0, A, Bo, Ypart};
-  launch_gemm2(A.conj_trans, false, dim3((m + GBM - 1) / GBM, (nn + GBN - 1) / GBN, nz), s, t, kchunk,
-               (size_t)m * nn);
-  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
 // ---------------------------------------------------------------------------

@@ -593,7 +593,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     if (j == nb - 1 || c == L - 2) {
       const int P = c / nb;
       const int cnt = toff[P + 1] - toff[P];
-      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, s))) return rc;
+      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, true, s))) return rc;
     }
   }
   hipLaunchKernelGGL(k_last_diag, dim3(1, 1, nprob), dim3(64), 0, s, M, ld, L, d, zb);

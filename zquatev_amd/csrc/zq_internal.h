@@ -400,12 +400,9 @@ struct QGemmTask {
 // their shapes; every task has the given operand conjugation flags.  Each task
 // computes exactly what qgemm would for it.
 int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool a_conj_trans, bool b_conj_trans,
-                hipStream_t s);
-// number of K chunks qgemm_cn_splitk uses for a given request
+                bool lower_only, hipStream_t s);
+// number of K chunks (and the chunk length) a split-K product over K uses
 int splitk_count(int K, int nsplit, int* kchunk_out);
-// Ypart[z] (m x nn, ld m) = A B over the z-th K chunk (A logical m x K, B K x nn)
-int qgemm_cn_splitk(int m, int nn, int K, const QOperand& A, const quat* B, int ldb, quat* Ypart, int nsplit,
-                    hipStream_t s);
 // real merge products of one divide & conquer tree level (see stedc.hip)
 int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const double* Qtmp,
                  const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s);
