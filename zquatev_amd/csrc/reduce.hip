@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
   }
   __shared__ quat cw[NBMAX], cv[NBMAX];
   __shared__ quat xs[RQ];
-  __shared__ quat red[16][RQ];
+  __shared__ double red[16][4][RQ];   // k-group, component, row
   __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
   const int r = tid & (RQ - 1), q = tid >> 4;
@@ -114,7 +114,10 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
       qfma(acc, wv, cv[l]);
     }
   }
-  red[q][r] = acc;
+  red[q][0][r] = acc.ar;
+  red[q][1][r] = acc.ai;
+  red[q][2][r] = acc.br;
+  red[q][3][r] = acc.bi;
   __syncthreads();
   double ss = 0.0;
   quat xm = qzero();
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld,
     if (j > 0) {
       quat sum = qzero();
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sum = qadd(sum, red[k][r]);
+      for (int k = 0; k < 16; ++k) sum = qadd(sum, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
       m = qsub(m, sum);
       qstore(M + i + (size_t)c * ld, m);
     }
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb); TSP = zoff(TSP, zb);
   }
   __shared__ double sh[SYT / 64];
-  __shared__ quat red[8][SG];
+  __shared__ double red[8][4][SG];   // wave, component, row: conflict-free b64 accesses
   const int s0 = c + 1;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -318,7 +321,10 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       if (ra > cg) qfma_cl(cp, a, ua);
       col[4 * cc + 0] = cp.ar; col[4 * cc + 1] = cp.ai; col[4 * cc + 2] = cp.br; col[4 * cc + 3] = cp.bi;
     }
-    red[w][lane] = acc;
+    red[w][0][lane] = acc.ar;
+    red[w][1][lane] = acc.ai;
+    red[w][2][lane] = acc.br;
+    red[w][3][lane] = acc.bi;
     // lane l ends with value (l >> 2) = component (l >> 2) & 3 of column (l >> 4), summed over the wave
     const double cs = butterfly_sum<16>(col);
     if ((lane & 3) == 0) {
@@ -332,7 +338,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     if (i < n && i >= C0) {
       quat sum = qzero();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sum = qadd(sum, red[k][tid]);
+      for (int k = 0; k < 8; ++k) sum = qadd(sum, quat{red[k][0][tid], red[k][1][tid], red[k][2][tid], red[k][3][tid]});
       qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
     scal = zoff(scal, zb); KP = zoff(KP, zb);
   }
   __shared__ quat a[NBMAX], b[NBMAX];
-  __shared__ quat red[16][RQ];
+  __shared__ double red[16][4][RQ];   // k-group, component, row
   __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
   const int s0 = c + 1;
@@ -398,13 +404,16 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
       acc = qsub(acc, qmul(ww[k], b[l]));
     }
   }
-  red[q][r] = acc;
+  red[q][0][r] = acc.ar;
+  red[q][1][r] = acc.ai;
+  red[q][2][r] = acc.br;
+  red[q][3][r] = acc.bi;
   __syncthreads();
   double kp = 0.0;
   if (q == 0 && i < n) {
     quat pv = qmul(ms0, qsub(u1, qscale(x1, gam)));
 #pragma unroll
-    for (int k = 0; k < 16; ++k) pv = qadd(pv, red[k][r]);
+    for (int k = 0; k < 16; ++k) pv = qadd(pv, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
     const quat ui = (i == s0) ? u1 : qscale(xi, gam);
     qstore(W + i + (size_t)j * ldw, pv);
     qstore(M + i + (size_t)c * ld, ui);

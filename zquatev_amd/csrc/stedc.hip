@@ -31,7 +31,7 @@
 
 namespace zq {
 
-constexpr int LEAF = 64;
+constexpr int LEAF = 32;
 constexpr double DEPS = DBL_EPSILON * 0.5;   // unit roundoff
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -74,7 +74,7 @@ __global__ void k_dc_tear(int nsplit, const int* __restrict__ splits, const doub
 }
 
 // --------------------------------------------------------------------------
-// leaf: implicit QL with Wilkinson-type shift on a <= 64 tridiagonal.  The
+// leaf: implicit QL with Wilkinson-type shift on a <= LEAF tridiagonal.  The
 // scalar recurrence runs identically in every lane; lane k owns row k of the
 // leaf's eigenvector block and applies each rotation to it.  A leaf is
 // (lo, hi, problem base, problem index).
@@ -569,10 +569,13 @@ __global__ void __launch_bounds__(256) k_bisect(const int* __restrict__ probs, c
   if (k >= n) return;
   d += o; e2 += o; w += o;
   double a = gl, b = gu;   // count(a) <= k < count(b)
+  // interval width down to a few ulps of the eigenvalue, or of the matrix
+  // norm (1 after scaling) for eigenvalues near zero: an absolute error of
+  // eps ||T|| is what any backward-stable eigensolver commits
   for (int it = 0; it < 200; ++it) {
     const double m = 0.5 * (a + b);
     if (!(m > a && m < b)) break;
-    if (b - a <= 2.0 * DEPS * fmax(fabs(a), fabs(b))) break;
+    if (b - a <= 2.0 * DEPS * fmax(fmax(fabs(a), fabs(b)), 1.0)) break;
     if (sturm_count(n, d, e2, m, pivmin) > k) b = m; else a = m;
   }
   w[k] = 0.5 * (a + b) * scal[blockIdx.y];
@@ -599,9 +602,9 @@ static void build_tree(int lo, int hi, int ob, int pr, int depth, std::vector<in
 }
 
 size_t stedc_int_slots(int ntot, int nprob) {
-  // per problem of size n at most 2 ceil(n / 64) leaves (4 ints), one split
-  // and one merge (4 ints) fewer, and (offset, size)
-  return (size_t)ntot / 3 + 24 * (size_t)nprob + 64;
+  // per problem of size n at most 2 ceil(n / LEAF) leaves (4 ints each), one
+  // split and one merge (4 ints) fewer, and (offset, size)
+  return (size_t)18 * (ntot / LEAF + nprob) + 2 * (size_t)nprob + 64;
 }
 
 int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const double* d, const double* e, double* w,
