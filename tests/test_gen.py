@@ -33,3 +33,14 @@ def test_workload_size_draws():
     assert s.min() >= 128 and s.max() <= 4096 and np.all(s % 2 == 0)
     # log-uniform: the median sits near the geometric mean of the range
     assert abs(np.log(np.median(s)) - 0.5 * (np.log(128) + np.log(4096))) < 0.1
+
+
+def test_uniform_case_device_is_bitwise_the_host_matrix():
+    n = 37
+    D = gen.uniform_case(n, 1004, diag_shift=10.0)
+    out = torch.zeros(2 * n, 2 * n, dtype=torch.complex128)
+    gen.uniform_case_device(n, 1004, out, diag_shift=10.0, chunk=8)
+    Dn = out.numpy().T
+    low = np.tril(np.ones((n, n), dtype=bool))
+    assert np.array_equal(Dn[:n, :n][low], D[:n, :n][low])
+    assert np.array_equal(Dn[n:, :n][low], D[n:, :n][low])

@@ -87,3 +87,52 @@ def test_zquatev_large(gpu_ctx, n2, nb):
         gpu_ctx.set_panel_width(0)
     assert info == 0
     inv.assert_solution(H, X, eig)
+
+
+def _device_checks(Dcols, n, eig, H, ncols=64):
+    """Size-independent checks of a device solve: trace identity, residual and
+    orthonormality of sampled eigenpairs, Kramers structure of the sampled columns."""
+    import torch
+    n2 = 2 * n
+    X = Dcols.T                                        # column-major view of the result
+    cols = torch.arange(0, n, max(1, n // ncols), device=H.device)
+    cols = torch.cat([cols, cols + n])
+    Xs = X[:, cols].contiguous()
+    lam = torch.cat([eig, eig])[cols].to(H.dtype)
+    g = torch.Generator(device=H.device).manual_seed(3)
+    v = torch.randn(n2, dtype=torch.float64, device=H.device, generator=g).to(H.dtype)
+    for _ in range(40):                                # power iteration: ||H||_2 from below
+        v = H @ v
+        v = v / torch.linalg.vector_norm(v)
+    hn = torch.linalg.vector_norm(H @ v).item()
+    res = (torch.linalg.norm(H @ Xs - Xs * lam[None, :]) / (n * hn)).item()
+    orth = (Xs.conj().T @ Xs - torch.eye(len(cols), dtype=Xs.dtype, device=H.device)).abs().max().item()
+    trace = abs(2.0 * eig.sum().item() - torch.diagonal(H).real.sum().item()) / (n2 * hn)
+    half = len(cols) // 2
+    U = Xs[:n, :half]
+    V = Xs[n:, :half]
+    struct = torch.equal(Xs[:n, half:], -V.conj()) and torch.equal(Xs[n:, half:], U.conj())
+    return res, orth, trace, struct, bool(torch.all(eig[1:] >= eig[:-1]).item())
+
+
+def test_zquatev_2n32768_large_solve(gpu_ctx):
+    """The large single solve at full size (2n = 32768, ~17 GB matrix resident
+    in HBM, uniform entries with diag(A) + 10): trace identity, residual and
+    orthonormality of 128 sampled eigenvectors, Kramers structure, ascending
+    eigenvalues."""
+    import torch
+    n = 16384
+    _say(f"case: large solve 2n={2 * n}")
+    Dcols = torch.zeros(2 * n, 2 * n, dtype=torch.complex128, device="cuda")
+    gen.uniform_case_device(n, 1004, Dcols, diag_shift=10.0)
+    H = gen.full_hermitian_t(Dcols, n)
+    eig = torch.zeros(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc = gpu_ctx.zquatev_device(2 * n, Dcols.data_ptr(), 2 * n, eig.data_ptr(), "V")
+    gpu_ctx.synchronize()
+    assert rc == 0
+    res, orth, trace, struct, ascending = _device_checks(Dcols, n, eig, H)
+    _say(f"  residual {res:.3e} orth {orth:.3e} trace {trace:.3e}")
+    assert res <= inv.RES_TOL and orth <= inv.ORTH_TOL and trace <= 1e-13 and struct and ascending
+    del H, Dcols
+    torch.cuda.empty_cache()

@@ -225,3 +225,26 @@ def serving_sizes(count, seed, lo=128, hi=4096):
     u = uniform(seed, np.arange(count))
     v = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
     return (2 * np.clip(np.round(v / 2.0), lo // 2, hi // 2)).astype(np.int64)
+
+
+def uniform_case_device(n, seed, out, diag_shift=0.0, chunk=256):
+    """:func:`uniform_case` written into ``out`` (torch complex128, out[j] =
+    column j of D, ld2 = out.shape[1]); only the lower triangles of A and B."""
+    import torch
+    dev = out.device
+    i = torch.arange(n, dtype=torch.int64, device=dev)[None, :]
+    zero = torch.zeros((), dtype=torch.float64, device=dev)
+    for j0 in range(0, n, chunk):
+        j1 = min(n, j0 + chunk)
+        j = torch.arange(j0, j1, dtype=torch.int64, device=dev)[:, None]
+        s = 4 * (j * n + i)
+        low = i > j
+        u0 = 2.0 * uniform_t(seed, s) - 1.0
+        a_re = torch.where(low, u0, zero)
+        a_im = torch.where(low, 2.0 * uniform_t(seed, s + 1) - 1.0, zero)
+        b_re = torch.where(low, 2.0 * uniform_t(seed, s + 2) - 1.0, zero)
+        b_im = torch.where(low, 2.0 * uniform_t(seed, s + 3) - 1.0, zero)
+        a_re = torch.where(i == j, u0 + diag_shift, a_re)
+        out[j0:j1, :n] = torch.complex(a_re, a_im)
+        out[j0:j1, n:2 * n] = torch.complex(b_re, b_im)
+    return out
