@@ -90,8 +90,8 @@ __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= t.total) return;
   quat s = qzero();
-  for (int z = 0; z < t.nz; ++z) s = qadd(s, qload(t.parts + (size_t)z * t.total + i));
-  qstore(t.out + i, s);
+  for (int z = 0; z < t.nz; ++z) s = qadd(s, qload_g(t.parts + (size_t)z * t.total + i));
+  qstore_g(t.out + i, s);
 }
 
 // T = (I + triu(G, 1))^{-1} for each block: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],

@@ -56,7 +56,7 @@ __device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, i
   if (k >= A.ksplit) { p = A.p1; ldp = A.ld1; kk = k - A.ksplit; }
   const int sr = CT ? kk : i, sc = CT ? i : kk;
   if (!(sr > sc + A.mask_off)) return qzero();
-  const quat v = qload(p + sr + (size_t)sc * ldp);
+  const quat v = qload_g(p + sr + (size_t)sc * ldp);
   return CT ? qconj(v) : v;
 }
 
@@ -68,7 +68,7 @@ __device__ __forceinline__ quat ld_opB(const QOperand& B, int k, int j, int K, i
   if (k >= B.ksplit) { p = B.p1; ldp = B.ld1; kk = k - B.ksplit; }
   const int sr = CT ? j : kk, sc = CT ? kk : j;
   if (!(sr > sc + B.mask_off)) return qzero();
-  const quat v = qload(p + sr + (size_t)sc * ldp);
+  const quat v = qload_g(p + sr + (size_t)sc * ldp);
   return CT ? qconj(v) : v;
 }
 
@@ -175,8 +175,8 @@ __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int co
       if (i < m && jj < nn && (!t.lower_only || i >= jj)) {
         quat* cp = C + i + (size_t)jj * t.ldc;
         quat r = quat{alpha * acc[a][b][0], alpha * acc[a][b][1], alpha * acc[a][b][2], alpha * acc[a][b][3]};
-        if (beta != 0.0) r = qadd(r, qscale(qload(cp), beta));
-        qstore(cp, r);
+        if (beta != 0.0) r = qadd(r, qscale(qload_g(cp), beta));
+        qstore_g(cp, r);
       }
     }
   }

@@ -48,8 +48,8 @@ __global__ void __launch_bounds__(256) k_load_left_b(const BatchProb* __restrict
   if (i >= n || j >= n || i < j) return;
   const int ld2 = P[blockIdx.z].ld2;
   const double2* col = reinterpret_cast<const double2*>(P[blockIdx.z].D) + (size_t)j * ld2;
-  const double2 a = col[i];
-  const double2 b = col[n + i];
+  const double2 a = d2load_g(col + i);
+  const double2 b = d2load_g(col + n + i);
   qstore(zoff(M, zb) + i + (size_t)j * ld, quat{a.x, a.y, b.x, b.y});
 }
 
@@ -63,10 +63,10 @@ __global__ void __launch_bounds__(256) k_store_vec_b(const BatchProb* __restrict
   const quat x = qload(zoff(X, zb) + i + (size_t)j * ldx);
   double2* left = reinterpret_cast<double2*>(P[blockIdx.z].D) + (size_t)j * ld2;
   double2* right = reinterpret_cast<double2*>(P[blockIdx.z].D) + (size_t)(j + n) * ld2;
-  left[i] = make_double2(x.ar, x.ai);
-  left[n + i] = make_double2(x.br, x.bi);
-  right[i] = make_double2(-x.br, x.bi);
-  right[n + i] = make_double2(x.ar, -x.ai);
+  d2store_g(left + i, x.ar, x.ai);
+  d2store_g(left + n + i, x.br, x.bi);
+  d2store_g(right + i, -x.br, x.bi);
+  d2store_g(right + n + i, x.ar, -x.ai);
 }
 
 __global__ void __launch_bounds__(256) k_gather_tri_b(const BatchProb* __restrict__ P, const double* __restrict__ d,
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_gather_tri_b(const BatchProb* __restric
 __global__ void __launch_bounds__(256) k_scatter_eig_b(const BatchProb* __restrict__ P, const double* __restrict__ wall) {
   const int n = P[blockIdx.z].n, off = P[blockIdx.z].off;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) P[blockIdx.z].eig[i] = wall[off + i];
+  if (i < n) dstore_g(P[blockIdx.z].eig + i, wall[off + i]);
 }
 
 __global__ void k_scatter_info_b(int nprob, const BatchProb* __restrict__ P, const int* __restrict__ info,

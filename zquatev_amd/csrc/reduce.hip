@@ -32,6 +32,9 @@ constexpr int SG = 64;       // matvec rows per block (one per lane)
 constexpr int SYT = 512;     // matvec threads: 8 waves x 4 columns
 constexpr int NBMAX = 128;   // largest supported panel width
 
+// ceil(a / b) for b > 0, and 0 for a <= 0
+__device__ __forceinline__ int cdiv_d(int a, int b) { return a > 0 ? (a + b - 1) / b : 0; }
+
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   v = wave_sum_d(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -219,7 +222,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
                                               int nparts, double* __restrict__ scal, double* __restrict__ d,
                                               quat* __restrict__ e, quat* __restrict__ Prow, quat* __restrict__ Pcol,
                                               int ldp, quat* __restrict__ TS, int nb, const double* __restrict__ TSP,
-                                              int ldtsp, ZB zb) {
+                                              int ldtsp, double* __restrict__ QF, ZB zb) {
   const int NT = j > 0 ? (8 * j + 63) / 64 : 0;
   const int NF = NT > 0 ? NT : 1;
   if (zb.nvec) {
@@ -229,8 +232,10 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     if ((int)blockIdx.x >= Gz * (Gz + 1) + NF) return;
     M = zoff(M, zb); W = zoff(W, zb); part = zoff(part, zb); scal = zoff(scal, zb); d = zoff(d, zb);
     e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb); TSP = zoff(TSP, zb);
+    QF = zoff(QF, zb);
   }
   __shared__ double sh[SYT / 64];
+  __shared__ double sqf[SYT / 64];
   __shared__ double red[8][4][SG];   // wave, component, row: conflict-free b64 accesses
   const int s0 = c + 1;
   const int tid = threadIdx.x;
@@ -257,6 +262,8 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       scal[0] = R.g;
       scal[1] = R.u1.ar; scal[2] = R.u1.ai; scal[3] = R.u1.br; scal[4] = R.u1.bi;
       scal[5] = x1.ar; scal[6] = x1.ai; scal[7] = x1.br; scal[8] = x1.bi;
+      // the trailing diagonal M[s0, s0] before k_wcol's update of column s0 overwrites it
+      scal[9] = M[s0 + (size_t)s0 * ld].ar;
       d[c] = M[c + (size_t)c * ld].ar;
       qstore(e + c, alpha);
     }
@@ -307,6 +314,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
 #pragma unroll
   for (int cc = 0; cc < 4; ++cc) xc[cc] = qload(M + min(C0 + 4 * w + cc, n - 1) + (size_t)c * ld);
   const quat ua = (ra < n) ? xa : qzero();
+  double qfl = 0.0;   // this thread's share of x^* M x over the tile
   if (active) {
     quat acc = qzero();
     double col[16];
@@ -315,7 +323,10 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       const int cl = 4 * w + cc;
       const int cg = C0 + cl;
       quat a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
-      if (ra == cg) a = quat{a.ar, 0.0, 0.0, 0.0};
+      if (ra == cg) {
+        a = quat{a.ar, 0.0, 0.0, 0.0};
+        qfl -= a.ar * qnorm2(ua);   // the diagonal enters x^* M x once, below twice
+      }
       qfma(acc, a, cg < n ? xc[cc] : qzero());
       quat cp = qzero();
       if (ra > cg) qfma_cl(cp, a, ua);
@@ -325,6 +336,8 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     red[w][1][lane] = acc.ai;
     red[w][2][lane] = acc.br;
     red[w][3][lane] = acc.bi;
+    // x^* M x over the tile's lower part, doubled off the diagonal: 2 Re(conj(x_i) acc_i)
+    qfl += 2.0 * (ua.ar * acc.ar + ua.ai * acc.ai + ua.br * acc.br + ua.bi * acc.bi);
     // lane l ends with value (l >> 2) = component (l >> 2) & 3 of column (l >> 4), summed over the wave
     const double cs = butterfly_sum<16>(col);
     if ((lane & 3) == 0) {
@@ -332,7 +345,15 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       if (cg < n) reinterpret_cast<double*>(Pcol + (size_t)g * ldp + cg)[(lane >> 2) & 3] = cs;
     }
   }
+  qfl = wave_sum_d(qfl);
+  if (lane == 0) sqf[w] = qfl;
   __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < SYT / 64; ++k) t += sqf[k];
+    QF[b] = t;
+  }
   if (active && tid < SG) {
     const int i = R0 + tid;
     if (i < n && i >= C0) {
@@ -344,30 +365,47 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   }
 }
 
-// p = y - V (W^* u) - W (V^* u) with y = M u = gamma (M x) + M[:, s0] (u1 - gamma x1),
-// (M x) from the matvec partials; stored in W[:, j].  Also stores u into
-// M[c+1:, c] and partial u^* p into KP.
+// Column c's w_j and, inside a panel, column c+1's pending update, in one
+// pass over rows s0 = c+1 .. n-1 (16 threads per row, KPT panel columns each).
+//   p   = y - V (W^* u) - W (V^* u),  y = M u = gamma (M x) + M[:, s0] (u1 - gamma x1)
+//         with (M x) from the matvec partials;
+//   K   = u^* p / 2 = (u^* y - 2 Re(a^* b)) / 2,  a = W^* u, b = V^* u (TS), and
+//         u^* y = gamma^2 x^* M x + 2 gamma Re(conj(delta) (M x)_{s0}) + M_{s0 s0} |delta|^2,
+//         delta = u1 - gamma x1, x^* M x from the matvec's per-tile partials (QF) --
+//         so every block has K without a grid-wide reduction of p;
+//   w_j = p - K u (final), stored in W[:, j]; u stored into M[s0:, c].
+// Unless column c closes its panel (j = nb - 1) or the reduction (c = n - 2),
+// the block then does for column c+1 what k_col_update does: subtract
+// [V W] [W V]^*-terms of the panel's columns 0..j (row c+1's coefficients,
+// including the new w_j(c+1) = p(s0) - K u1, are formed redundantly in every
+// block), store the column, and leave its tail-norm partials in part and the
+// per-16-row partials of W^* x, V^* x in TSP.
 template <int KPT>
-__global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, int n, int p, int c, int j,
-                                                quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
-                                                const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
-                                                int nb, const double* __restrict__ scal,
-                                                double* __restrict__ KP, ZB zb) {
+__global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n, int p, int c, int j,
+                                             quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
+                                             const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
+                                             int nb, const double* __restrict__ scal, const double* __restrict__ QF,
+                                             double* __restrict__ part, double* __restrict__ TSP, int ldtsp, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     if (c + 1 + (int)blockIdx.x * RQ >= n) return;
     M = zoff(M, zb); W = zoff(W, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb);
-    scal = zoff(scal, zb); KP = zoff(KP, zb);
+    scal = zoff(scal, zb); QF = zoff(QF, zb); part = zoff(part, zb); TSP = zoff(TSP, zb);
   }
-  __shared__ quat a[NBMAX], b[NBMAX];
-  __shared__ double red[16][4][RQ];   // k-group, component, row
+  __shared__ quat a[NBMAX], b[NBMAX], cw[NBMAX], cv[NBMAX];
+  __shared__ double red[16][8][RQ];   // k-group, (p | column update) component, row
+  __shared__ double shr[RB / 64][10];
+  __shared__ quat xs[RQ], us[RQ], ws[RQ];
   __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const int s0 = c + 1;
+  const bool more = (j + 1 < nb) && (c + 1 <= n - 2);   // uniform
   const int r = tid & (RQ - 1), q = tid >> 4;
   const int i = s0 + blockIdx.x * RQ + r;
   const int ic = min(i, n - 1);
   const int jl = j > 0 ? j - 1 : 0;
+  // ---- loads, all issued up front
   quat vv[KPT], ww[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
@@ -376,77 +414,171 @@ __global__ void __launch_bounds__(RB) k_wreduce(quat* __restrict__ M, int ld, in
     ww[k] = qload(W + ic + (size_t)l * ldw);
   }
   const quat xi = qload(M + ic + (size_t)c * ld);
-  quat ms0 = qload(M + ic + (size_t)s0 * ld);
-  if (ic == s0) ms0 = quat{ms0.ar, 0.0, 0.0, 0.0};   // Hermitian diagonal: real
+  const quat ms0raw = qload(M + ic + (size_t)s0 * ld);   // stale column s0 = c+1 (row i)
+  const double m00 = scal[9];   // M[s0, s0] itself is rewritten by this grid (row s0 of column s0)
   const double gam = scal[0];
   const quat u1 = quat{scal[1], scal[2], scal[3], scal[4]};
   const quat x1 = quat{scal[5], scal[6], scal[7], scal[8]};
-  quat yi = qzero();
+  const int G = (n - s0 + SG - 1) / SG;
+  // The matvec partials this thread sums, gathered in fixed-size batches of
+  // predicated loads so that one wait covers them all (one batch up to
+  // trailing sizes of ~4k rows):
+  //   y(i):  slots k = q + 16 t of row i's list, Prow[J][i] for J <= li / SW
+  //          (k < nJ), then Pcol[g][i] for g >= li / SG;
+  //   x^* M x: QF[tid + RB t], t over the G (G + 1) tile partials;
+  //   (M x)_{s0}: Prow[0][s0] and Pcol[g][s0] as slots tid + RB t of G + 1.
+  constexpr int PY = 10, PQ = 18;
+  const int li = ic - s0;
+  const int nJ = li / SW + 1, g0 = li / SG;
+  const int cy = cdiv_d(nJ + G - g0 - q, 16);
+  const int nq = G * (G + 1);
+  const int cq = cdiv_d(nq - tid, RB);
+  const int cw0 = cdiv_d(G + 1 - tid, RB);
+  quat yi = qzero(), w0 = qzero();
+  double qf = 0.0;
+  for (int bt = 0; bt * PY < cy || bt * PQ < cq || bt < cw0; ++bt) {
+    quat py[PY];
+    double pq[PQ];
+#pragma unroll
+    for (int u = 0; u < PY; ++u) {
+      const int t = bt * PY + u;
+      const int k = q + 16 * t;
+      const quat* src = k < nJ ? Prow + (size_t)k * ldp + ic : Pcol + (size_t)(g0 + k - nJ) * ldp + ic;
+      py[u] = t < cy ? qload(src) : qzero();
+    }
+#pragma unroll
+    for (int u = 0; u < PQ; ++u) {
+      const int t = bt * PQ + u;
+      pq[u] = t < cq ? QF[tid + RB * t] : 0.0;
+    }
+    const int kw = tid + RB * bt;
+    const quat pw = bt < cw0 ? qload(kw == 0 ? Prow + s0 : Pcol + (size_t)(kw - 1) * ldp + s0) : qzero();
+#pragma unroll
+    for (int u = 0; u < PY; ++u) yi = qadd(yi, py[u]);
+#pragma unroll
+    for (int u = 0; u < PQ; ++u) qf += pq[u];
+    w0 = qadd(w0, pw);
+  }
+  quat al = qzero(), bl = qzero(), cwl = qzero(), cvl = qzero();
+  if (tid < j) {
+    al = qload(TS + tid);
+    bl = qload(TS + nb + tid);
+    cwl = qload(W + s0 + (size_t)tid * ldw);
+    cvl = qload(M + s0 + (size_t)(p + tid) * ld);
+  }
+  // ---- block sums: x^* M x, Re(a^* b), (M x)_{s0}, V(s0,:) a + W(s0,:) b
   {
-    const int li = ic - s0;
-    const int G = (n - s0 + SG - 1) / SG;
-#pragma unroll 4
-    for (int J = q; J <= li / SW; J += 16) yi = qadd(yi, qload(Prow + (size_t)J * ldp + ic));
-#pragma unroll 4
-    for (int gg = li / SG + q; gg < G; gg += 16) yi = qadd(yi, qload(Pcol + (size_t)gg * ldp + ic));
+    quat rd = qmul(cvl, al);
+    qfma(rd, cwl, bl);
+    double v[10] = {qf, al.ar * bl.ar + al.ai * bl.ai + al.br * bl.br + al.bi * bl.bi,
+                    w0.ar, w0.ai, w0.br, w0.bi, rd.ar, rd.ai, rd.br, rd.bi};
+#pragma unroll
+    for (int t = 0; t < 10; ++t) v[t] = wave_sum_d(v[t]);
+    if (lane == 0)
+#pragma unroll
+      for (int t = 0; t < 10; ++t) shr[wv][t] = v[t];
   }
   if (tid < j) {
-    a[tid] = qload(TS + tid);
-    b[tid] = qload(TS + nb + tid);
+    a[tid] = al;
+    b[tid] = bl;
+    cw[tid] = qconj(cwl);
+    cv[tid] = qconj(cvl);
   }
   __syncthreads();
-  quat acc = qscale(yi, gam);   // this thread's share of y(i) minus its share of the corrections
+  double S[10];
+#pragma unroll
+  for (int t = 0; t < 10; ++t) {
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < RB / 64; ++k) x += shr[k][t];
+    S[t] = x;
+  }
+  const quat delta = qsub(u1, qscale(x1, gam));
+  const quat wrow = quat{S[2], S[3], S[4], S[5]};   // (M x)_{s0}
+  const double uy = gam * gam * S[0] +
+                    2.0 * gam * (delta.ar * wrow.ar + delta.ai * wrow.ai + delta.br * wrow.br + delta.bi * wrow.bi) +
+                    m00 * qnorm2(delta);
+  const double K = 0.5 * (uy - 2.0 * S[1]);
+  // p and w_j at row s0 (= c+1), needed by every block for column c+1's update
+  const quat ps0 = qsub(qadd(qscale(wrow, gam), qscale(delta, m00)), quat{S[6], S[7], S[8], S[9]});
+  const quat wc1 = qsub(ps0, qscale(u1, K));
+  // ---- this thread's shares: y(i) minus corrections, and column c+1's update over l < j
+  quat accp = qscale(yi, gam), accu = qzero();
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int l = q + 16 * k;
     if (l < j) {
-      acc = qsub(acc, qmul(vv[k], a[l]));
-      acc = qsub(acc, qmul(ww[k], b[l]));
+      accp = qsub(accp, qmul(vv[k], a[l]));
+      accp = qsub(accp, qmul(ww[k], b[l]));
+      if (more) {
+        qfma(accu, vv[k], cw[l]);
+        qfma(accu, ww[k], cv[l]);
+      }
     }
   }
-  red[q][0][r] = acc.ar;
-  red[q][1][r] = acc.ai;
-  red[q][2][r] = acc.br;
-  red[q][3][r] = acc.bi;
+  red[q][0][r] = accp.ar; red[q][1][r] = accp.ai; red[q][2][r] = accp.br; red[q][3][r] = accp.bi;
+  red[q][4][r] = accu.ar; red[q][5][r] = accu.ai; red[q][6][r] = accu.br; red[q][7][r] = accu.bi;
   __syncthreads();
-  double kp = 0.0;
-  if (q == 0 && i < n) {
-    quat pv = qmul(ms0, qsub(u1, qscale(x1, gam)));
+  double ss = 0.0;
+  if (q == 0) {
+    quat xn = qzero(), ui = qzero(), wi = qzero();
+    if (i < n) {
+      const quat ms0 = (i == s0) ? quat{ms0raw.ar, 0.0, 0.0, 0.0} : ms0raw;   // Hermitian diagonal: real
+      quat pv = qmul(ms0, delta), su = qzero();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) pv = qadd(pv, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
-    const quat ui = (i == s0) ? u1 : qscale(xi, gam);
-    qstore(W + i + (size_t)j * ldw, pv);
-    qstore(M + i + (size_t)c * ld, ui);
-    kp = ui.ar * pv.ar + ui.ai * pv.ai + ui.br * pv.br + ui.bi * pv.bi;
+      for (int k = 0; k < 16; ++k) {
+        pv = qadd(pv, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
+        su = qadd(su, quat{red[k][4][r], red[k][5][r], red[k][6][r], red[k][7][r]});
+      }
+      ui = (i == s0) ? u1 : qscale(xi, gam);
+      wi = qsub(pv, qscale(ui, K));
+      qstore(W + i + (size_t)j * ldw, wi);
+      qstore(M + i + (size_t)c * ld, ui);
+      if (more) {
+        // column c+1, row i: stale value minus the panel's columns 0..j
+        xn = qsub(ms0raw, su);
+        xn = qsub(xn, qmul(ui, qconj(wc1)));
+        xn = qsub(xn, qmul(wi, qconj(u1)));
+        qstore(M + i + (size_t)s0 * ld, xn);
+        if (i >= s0 + 2) ss = qnorm2(xn);
+        else xn = qzero();
+      }
+    }
+    xs[r] = xn;
+    us[r] = ui;
+    ws[r] = wi;
   }
-  kp = block_sum(kp, sh);
-  if (tid == 0) KP[blockIdx.x] = kp;
-}
-
-// w = p - (u^* p / 2) u  (last column of a panel; the others are finished by
-// the next column's k_col_update).  In a batch, only the problems whose panel
-// ends at column c (j = nb - 1, or c = n - 2) take part.
-__global__ void __launch_bounds__(RB) k_wfinal(const quat* __restrict__ M, int ld, int n, int c, int j,
-                                               quat* __restrict__ W, int ldw, const double* __restrict__ KP,
-                                               int nkp, ZB zb, int nb) {
-  if (zb.nvec) {
-    n = zb.nvec[blockIdx.z];
-    if (!(j == nb - 1 || c == n - 2)) return;
-    if (c + 1 + (int)blockIdx.x * RB >= n) return;
-    nkp = (n - c - 1 + RQ - 1) / RQ;
-    M = zoff(M, zb); W = zoff(W, zb); KP = zoff(KP, zb);
-  }
-  __shared__ double sh[RB / 64];
-  double t = 0.0;
-  for (int i = threadIdx.x; i < nkp; i += RB) t += KP[i];
-  t = block_sum(t, sh);
-  const double K = 0.5 * t;
-  const int i = c + 1 + blockIdx.x * RB + threadIdx.x;
-  if (i < n) {
-    const quat u = qload(M + i + (size_t)c * ld);
-    quat y = qload(W + i + (size_t)j * ldw);
-    y = qsub(y, qscale(u, K));
-    qstore(W + i + (size_t)j * ldw, y);
+  if (!more) return;   // uniform
+  ss = block_sum(ss, sh);   // its barriers also publish xs, us, ws
+  if (tid == 0) part[blockIdx.x] = ss;
+  // tall-skinny partials of column c+1 over the panel columns 0..j (j + 1 of them):
+  // this thread's columns l = q + 16 k against x(row r), summed over the 16 rows
+  {
+    constexpr int NV = (8 * KPT < 16) ? 16 : 8 * KPT;
+    double v[NV];
+#pragma unroll
+    for (int t = 0; t < NV; ++t) v[t] = 0.0;
+    const quat x = xs[r];
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int l = q + 16 * k;
+      if (l <= j) {
+        const quat wl = (l == j) ? ws[r] : ww[k];
+        const quat vl = (l == j) ? us[r] : vv[k];
+        quat ta = qzero(), tb = qzero();
+        qfma_cl(ta, wl, x);
+        qfma_cl(tb, vl, x);
+        v[8 * k + 0] = ta.ar; v[8 * k + 1] = ta.ai; v[8 * k + 2] = ta.br; v[8 * k + 3] = ta.bi;
+        v[8 * k + 4] = tb.ar; v[8 * k + 5] = tb.ai; v[8 * k + 6] = tb.br; v[8 * k + 7] = tb.bi;
+      }
+    }
+    row16_halving<NV>(v);
+#pragma unroll
+    for (int t = 0; t < NV / 16; ++t) {
+      const int idx = r * (NV / 16) + t;
+      const int k = idx >> 3, l = q + 16 * k;
+      if (k < KPT && l <= j) TSP[(size_t)blockIdx.x * ldtsp + 8 * l + (idx & 7)] = v[t];
+    }
   }
 }
 
@@ -477,15 +609,18 @@ static void launch_col_update(int kpt, dim3 grid, hipStream_t s, Args... args) {
 }
 
 template <class... Args>
-static void launch_wreduce(int kpt, dim3 grid, hipStream_t s, Args... args) {
+static void launch_wcol(int kpt, dim3 grid, hipStream_t s, Args... args) {
   switch (kpt) {
-    case 1: hipLaunchKernelGGL(k_wreduce<1>, grid, dim3(RB), 0, s, args...); break;
-    case 2: hipLaunchKernelGGL(k_wreduce<2>, grid, dim3(RB), 0, s, args...); break;
-    case 4: hipLaunchKernelGGL(k_wreduce<4>, grid, dim3(RB), 0, s, args...); break;
-    default: hipLaunchKernelGGL(k_wreduce<8>, grid, dim3(RB), 0, s, args...); break;
+    case 1: hipLaunchKernelGGL(k_wcol<1>, grid, dim3(RB), 0, s, args...); break;
+    case 2: hipLaunchKernelGGL(k_wcol<2>, grid, dim3(RB), 0, s, args...); break;
+    case 4: hipLaunchKernelGGL(k_wcol<4>, grid, dim3(RB), 0, s, args...); break;
+    default: hipLaunchKernelGGL(k_wcol<8>, grid, dim3(RB), 0, s, args...); break;
   }
 }
 
+// Per panel: k_col_update for its first column, then per column k_symv and
+// k_wcol (which also updates the next column of the panel); then the
+// trailing update.
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s) {
   if (n <= 0) return ZQI_OK;
   if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
@@ -495,16 +630,14 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
   }
   const int ldw = n;
   const int kpt = kpt_for(nb);
-  int panel = 0;
-  for (int p = 0; p < n - 1; p += nb, ++panel) {
+  for (int p = 0; p < n - 1; p += nb) {
     const int wdt = (nb < n - 1 - p) ? nb : (n - 1 - p);
     for (int j = 0; j < wdt; ++j) {
       const int c = p + j;
-      const int nrow = n - c;                      // rows c..n-1
-      const int nblk = cdiv(nrow, RQ);
-      const int nkp = cdiv(nrow, RQ);              // blocks of the previous column's k_wreduce
-      launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, nkp, w.part, w.TSP, w.ldtsp,
-                        ZB{nullptr, 0});
+      const int nblk = cdiv(n - c, RQ);            // row blocks from c
+      if (j == 0)
+        launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, 0, w.part, w.TSP, w.ldtsp,
+                          ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
       const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
@@ -512,14 +645,10 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
         launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
-                 w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, ZB{nullptr, 0});
+                 w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
       }
-      const int nb2 = cdiv(sz, RQ);
-      launch_wreduce(kpt, dim3(nb2), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal, w.KP,
-                     ZB{nullptr, 0});
-      if (j == wdt - 1)
-        hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB)), dim3(RB), 0, s, M, ld, n, c, j, w.W, ldw, w.KP, nb2,
-                           ZB{nullptr, 0}, nb);
+      launch_wcol(kpt, dim3(cdiv(sz, RQ)), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal,
+                  w.QF, w.part, w.TSP, w.ldtsp, ZB{nullptr, 0});
     }
     // trailing update of rows/cols [p+wdt, n): M -= [V W] [W V]^*  (lower triangle)
     const int r0 = p + wdt;
@@ -582,23 +711,16 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     const int p = (c / nb) * nb, j = c - p;
     const int nrow = L - c;
     const int nblk = cdiv(nrow, RQ);
-    launch_col_update(kpt, dim3(nblk, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.KP, nblk, w.part, w.TSP, w.ldtsp,
-                      zb);
+    if (j == 0)
+      launch_col_update(kpt, dim3(nblk, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.KP, 0, w.part, w.TSP, w.ldtsp,
+                        zb);
     const int sz = L - c - 1;
     const int G = cdiv(sz, SG);
     const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
     hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
-                       w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, zb);
-    const int nb2 = cdiv(sz, RQ);
-    launch_wreduce(kpt, dim3(nb2, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb, w.scal,
-                   w.KP, zb);
-    // w of the last column of a panel: every active problem at a panel end,
-    // else those whose last column is c
-    bool fin = (j == nb - 1);
-    for (int z = 0; z < act && !fin; ++z) fin = (nh[z] - 2 == c);
-    if (fin)
-      hipLaunchKernelGGL(k_wfinal, dim3(cdiv(sz, RB), 1, act), dim3(RB), 0, s, M, ld, L, c, j, w.W, ldw, w.KP, nb2,
-                         zb, nb);
+                       w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, w.QF, zb);
+    launch_wcol(kpt, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
+                w.scal, w.QF, w.part, w.TSP, w.ldtsp, zb);
     if (j == nb - 1 || c == L - 2) {
       const int P = c / nb;
       const int cnt = toff[P + 1] - toff[P];

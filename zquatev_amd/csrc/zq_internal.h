@@ -11,6 +11,7 @@
 //     conjugate (a,b)^* = (conj a, -b); |q|^2 = |a|^2 + |b|^2.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <hip/hip_ext.h>
 #include <cstdint>
 #include <cstddef>
@@ -34,6 +35,34 @@ __device__ __forceinline__ void qstore(quat* p, const quat& v) {
   double2* q = reinterpret_cast<double2*>(p);
   q[0] = make_double2(v.ar, v.ai);
   q[1] = make_double2(v.br, v.bi);
+}
+
+// Loads and stores through pointers read from tables in device memory (task
+// and problem tables): the compiler cannot tell their address space and would
+// emit flat instructions; every one of them points to global memory.  The
+// accesses go through global-space pointers to scalars or native vectors (a
+// struct copy would become a generic memcpy).
+#define ZQ_GLOBAL __attribute__((address_space(1)))
+typedef double zq_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 d2load_g(const double2* p) {
+  const zq_d2 v = *(const ZQ_GLOBAL zq_d2*)p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void d2store_g(double2* p, double x, double y) {
+  const zq_d2 v = {x, y};
+  *(ZQ_GLOBAL zq_d2*)p = v;
+}
+__device__ __forceinline__ void dstore_g(double* p, double x) { *(ZQ_GLOBAL double*)p = x; }
+__device__ __forceinline__ quat qload_g(const quat* p) {
+  const ZQ_GLOBAL zq_d2* g = (const ZQ_GLOBAL zq_d2*)p;
+  const zq_d2 x = g[0], y = g[1];
+  return quat{x.x, x.y, y.x, y.y};
+}
+__device__ __forceinline__ void qstore_g(quat* p, const quat& v) {
+  ZQ_GLOBAL zq_d2* g = (ZQ_GLOBAL zq_d2*)p;
+  const zq_d2 x = {v.ar, v.ai}, y = {v.br, v.bi};
+  g[0] = x;
+  g[1] = y;
 }
 
 __device__ __forceinline__ quat qadd(const quat& x, const quat& y) {
@@ -240,9 +269,13 @@ struct ZB {
   const int* nvec;
   size_t zs;
 };
+// Byte arithmetic on the pointer itself (no round trip through an integer),
+// so the compiler keeps kernel arguments in the global address space and
+// emits global rather than flat memory instructions.
 template <class T>
 __device__ __forceinline__ T* zoff(T* p, const ZB& zb) {
-  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zb.zs);
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(p) + (size_t)blockIdx.z * zb.zs);
 }
 
 // reduce.hip: quaternion Householder tridiagonalisation of the lower triangle
@@ -255,6 +288,7 @@ struct ReduceWork {
   quat* TS;       // tall-skinny products [W^* u | V^* u]: 2 x nb
   double* TSP;    // their per-block partials: ceil(n/16) x ldtsp
   int ldtsp;      // 8 nb
+  double* QF;     // per-tile partials of x^* M x: ceil(n/64) (ceil(n/64) + 1)
   double* KP;     // ceil(n/16)
   double* part;   // ceil(n/16)
   double* scal;   // 16
