@@ -191,7 +191,6 @@ void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   p->rw.ldtsp = 8 * nb;
   p->rw.TSP = b.take<double>((size_t)(cdiv(nmax, 16) + 1) * 8 * nb);
   p->rw.QF = b.take<double>((size_t)(cdiv(nmax, 64) + 1) * (cdiv(nmax, 64) + 2));
-  p->rw.KP = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.part = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.scal = b.take<double>(16);
   if (vectors) {

@@ -47,126 +47,22 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Column c of the panel (index j): apply the panel's pending update
-//   M[i,c] -= sum_{l<j} V(i,l) W(c,l)^* + W(i,l) V(c,l)^*     for i in [c, n)
-// and accumulate |M[i,c]|^2 over i >= c+2 (the reflector's tail norm).
-// Also finishes w_{j-1} = p_{j-1} - K u_{c-1} (K = u^*p / 2 from the previous
-// column's partials): each W(i, j-1), i > c, is read and rewritten by the one
-// thread that uses it; W(c, j-1) is formed on the fly (no later kernel reads
-// it).  Finally the block's 16 rows of the tall-skinny products W^* x and
-// V^* x (x the updated column, rows >= c+2) go to TSP[block] (the matvec
-// kernel adds them up in a fixed order).
-// 16 threads per row, KPT panel columns each; every load is issued up front
-// at a clamped address.
-template <int KPT>
-__global__ void __launch_bounds__(RB) k_col_update(quat* __restrict__ M, int ld, int n, int p, int c, int j,
-                                                   quat* __restrict__ W, int ldw, const double* __restrict__ KP,
-                                                   int nkp, double* __restrict__ part, double* __restrict__ TSP,
-                                                   int ldtsp, ZB zb) {
+// Tail-norm partials of the first column c of a panel (no update is pending
+// there): part[b] = sum of |M[i, c]|^2 over rows i >= c + 2 of the row block
+// [c + RQ b, c + RQ (b + 1)) -- the row blocks in which k_wcol leaves them for
+// every later column of the panel.  Each 256-thread block covers 16 row blocks.
+__global__ void __launch_bounds__(256) k_col_norm(const quat* __restrict__ M, int ld, int n, int c,
+                                                  double* __restrict__ part, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
-    if (c + (int)blockIdx.x * RQ >= n) return;
-    nkp = (n - c + RQ - 1) / RQ;
-    M = zoff(M, zb); W = zoff(W, zb); KP = zoff(KP, zb); part = zoff(part, zb); TSP = zoff(TSP, zb);
+    M = zoff(M, zb); part = zoff(part, zb);
   }
-  __shared__ quat cw[NBMAX], cv[NBMAX];
-  __shared__ quat xs[RQ];
-  __shared__ double red[16][4][RQ];   // k-group, component, row
-  __shared__ double sh[RB / 64];
-  const int tid = threadIdx.x;
-  const int r = tid & (RQ - 1), q = tid >> 4;
-  const int i = c + blockIdx.x * RQ + r;
-  const int ic = min(i, n - 1);
-  const int jl = j > 0 ? j - 1 : 0;
-  const quat mcur = qload(M + ic + (size_t)c * ld);
-  const quat uprev = qload(M + ic + (size_t)(c > 0 ? c - 1 : 0) * ld);
-  quat vv[KPT], ww[KPT];
+  const int i = c + blockIdx.x * 256 + threadIdx.x;
+  double v = (i >= c + 2 && i < n) ? qnorm2(qload(M + i + (size_t)c * ld)) : 0.0;
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int l = min(q + 16 * k, jl);
-    vv[k] = qload(M + ic + (size_t)(p + l) * ld);
-    ww[k] = qload(W + ic + (size_t)l * ldw);
-  }
-  double t = 0.0;
-  for (int k = tid; k < nkp; k += RB) t += KP[k];
-  quat cwl = qzero(), cvl = qzero(), ucc = qzero();
-  if (tid < j) {
-    cwl = qload(W + c + (size_t)tid * ldw);
-    cvl = qload(M + c + (size_t)(p + tid) * ld);
-    if (tid == j - 1) ucc = qload(M + c + (size_t)(c - 1) * ld);
-  }
-  const double K = (j > 0) ? 0.5 * block_sum(t, sh) : 0.0;
-  if (tid < j) {
-    if (tid == j - 1) cwl = qsub(cwl, qscale(ucc, K));
-    cw[tid] = qconj(cwl);
-    cv[tid] = qconj(cvl);
-  }
-  __syncthreads();
-  quat acc = qzero();
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int l = q + 16 * k;
-    if (l < j) {
-      quat wv = ww[k];
-      if (l == j - 1) {
-        wv = qsub(wv, qscale(uprev, K));
-        if (i > c && i < n) qstore(W + i + (size_t)l * ldw, wv);
-        ww[k] = wv;
-      }
-      qfma(acc, vv[k], cw[l]);
-      qfma(acc, wv, cv[l]);
-    }
-  }
-  red[q][0][r] = acc.ar;
-  red[q][1][r] = acc.ai;
-  red[q][2][r] = acc.br;
-  red[q][3][r] = acc.bi;
-  __syncthreads();
-  double ss = 0.0;
-  quat xm = qzero();
-  if (q == 0 && i < n) {
-    quat m = mcur;
-    if (j > 0) {
-      quat sum = qzero();
-#pragma unroll
-      for (int k = 0; k < 16; ++k) sum = qadd(sum, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
-      m = qsub(m, sum);
-      qstore(M + i + (size_t)c * ld, m);
-    }
-    if (i >= c + 2) {
-      ss = qnorm2(m);
-      xm = m;
-    }
-  }
-  if (q == 0) xs[r] = xm;
-  ss = block_sum(ss, sh);   // its barriers also publish xs
-  if (tid == 0) part[blockIdx.x] = ss;
-  if (j > 0) {
-    // this thread's panel columns l = q + 16 k against x_i of its row, then
-    // summed over the 16 rows (the 16 lanes sharing q) by a halving butterfly
-    constexpr int NV = (8 * KPT < 16) ? 16 : 8 * KPT;
-    double v[NV];
-#pragma unroll
-    for (int t = 0; t < NV; ++t) v[t] = 0.0;
-    const quat x = xs[r];
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      if (q + 16 * k < j) {
-        quat a = qzero(), b = qzero();
-        qfma_cl(a, ww[k], x);
-        qfma_cl(b, vv[k], x);
-        v[8 * k + 0] = a.ar; v[8 * k + 1] = a.ai; v[8 * k + 2] = a.br; v[8 * k + 3] = a.bi;
-        v[8 * k + 4] = b.ar; v[8 * k + 5] = b.ai; v[8 * k + 6] = b.br; v[8 * k + 7] = b.bi;
-      }
-    }
-    row16_halving<NV>(v);
-#pragma unroll
-    for (int t = 0; t < NV / 16; ++t) {
-      const int idx = r * (NV / 16) + t;
-      const int k = idx >> 3, l = q + 16 * k;
-      if (k < KPT && l < j) TSP[(size_t)blockIdx.x * ldtsp + 8 * l + (idx & 7)] = v[t];
-    }
-  }
+  for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
+  const int pb = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if ((threadIdx.x & 15) == 0 && c + pb * RQ < n) part[pb] = v;
 }
 
 // Reflector for column c: x = M[c+1:, c].  H x = alpha e_1 with
@@ -200,8 +96,10 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
 // M[s0:, s0:] (s0 = c + 1), applied to the raw updated column x = M[s0:, c]
 // rather than to the reflector u: u = gamma x + (u1 - gamma x1) e_1, so
 //   M u = gamma (M x) + (u1 - gamma x1) M[s0:, s0],
-// which k_wreduce forms row by row.  The tile blocks therefore never wait for
-// the reflector (no norm reduction on their path).
+// which k_wcol forms row by row.  The tile blocks therefore never wait for
+// the reflector (no norm reduction on their path).  Each tile block also
+// leaves its share of x^* M x (the lower part doubled off the diagonal) in
+// QF[b], from which k_wcol forms u^* M u without another grid-wide reduction.
 // Block (g, J) reads rows [s0 + SG g, +SG) x columns [s0 + SW J, +SW) (every
 // lower-triangle element exactly once over the grid); wave w of 8 takes 4 of
 // the 32 columns, one row per lane, all loads issued up front.  Row sums over
@@ -215,7 +113,7 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
 // reflector from the tail-norm partials (the first of them publishes gamma,
 // u1, x1 in scal and d[c], e[c]) and finish the tall-skinny products
 // TS = [W^* u | V^* u]: they add the per-block partials of W^* x and V^* x
-// that k_col_update left in TSP (8 waves over the partial index, then the 8
+// that k_wcol left in TSP (8 waves over the partial index, then the 8
 // wave sums, in a fixed order), scale by gamma and add the row s0 term.
 __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                               const quat* __restrict__ W, int ldw, const double* __restrict__ part,
@@ -375,7 +273,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
 //         so every block has K without a grid-wide reduction of p;
 //   w_j = p - K u (final), stored in W[:, j]; u stored into M[s0:, c].
 // Unless column c closes its panel (j = nb - 1) or the reduction (c = n - 2),
-// the block then does for column c+1 what k_col_update does: subtract
+// the block then applies column c+1's pending update: subtract the
 // [V W] [W V]^*-terms of the panel's columns 0..j (row c+1's coefficients,
 // including the new w_j(c+1) = p(s0) - K u1, are formed redundantly in every
 // block), store the column, and leave its tail-norm partials in part and the
@@ -419,6 +317,13 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
   const double gam = scal[0];
   const quat u1 = quat{scal[1], scal[2], scal[3], scal[4]};
   const quat x1 = quat{scal[5], scal[6], scal[7], scal[8]};
+  quat al = qzero(), bl = qzero(), cwl = qzero(), cvl = qzero();
+  if (tid < j) {
+    al = qload(TS + tid);
+    bl = qload(TS + nb + tid);
+    cwl = qload(W + s0 + (size_t)tid * ldw);
+    cvl = qload(M + s0 + (size_t)(p + tid) * ld);
+  }
   const int G = (n - s0 + SG - 1) / SG;
   // The matvec partials this thread sums, gathered in fixed-size batches of
   // predicated loads so that one wait covers them all (one batch up to
@@ -459,13 +364,6 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
     for (int u = 0; u < PQ; ++u) qf += pq[u];
     w0 = qadd(w0, pw);
   }
-  quat al = qzero(), bl = qzero(), cwl = qzero(), cvl = qzero();
-  if (tid < j) {
-    al = qload(TS + tid);
-    bl = qload(TS + nb + tid);
-    cwl = qload(W + s0 + (size_t)tid * ldw);
-    cvl = qload(M + s0 + (size_t)(p + tid) * ld);
-  }
   // ---- block sums: x^* M x, Re(a^* b), (M x)_{s0}, V(s0,:) a + W(s0,:) b
   {
     quat rd = qmul(cvl, al);
@@ -503,17 +401,22 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
   const quat ps0 = qsub(qadd(qscale(wrow, gam), qscale(delta, m00)), quat{S[6], S[7], S[8], S[9]});
   const quat wc1 = qsub(ps0, qscale(u1, K));
   // ---- this thread's shares: y(i) minus corrections, and column c+1's update over l < j
+  // branch-free (reads at a clamped index, results selected) so that every
+  // LDS read is issued before the first is used
   quat accp = qscale(yi, gam), accu = qzero();
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int l = q + 16 * k;
-    if (l < j) {
-      accp = qsub(accp, qmul(vv[k], a[l]));
-      accp = qsub(accp, qmul(ww[k], b[l]));
-      if (more) {
-        qfma(accu, vv[k], cw[l]);
-        qfma(accu, ww[k], cv[l]);
-      }
+    const bool on = l < j;
+    const int lc = on ? l : 0;
+    const quat ak = a[lc], bk = b[lc], cwk = cw[lc], cvk = cv[lc];
+    quat tp = qmul(vv[k], ak);
+    qfma(tp, ww[k], bk);
+    quat tu = qmul(vv[k], cwk);
+    qfma(tu, ww[k], cvk);
+    if (on) {
+      accp = qsub(accp, tp);
+      accu = qadd(accu, tu);
     }
   }
   red[q][0][r] = accp.ar; red[q][1][r] = accp.ai; red[q][2][r] = accp.br; red[q][3][r] = accp.bi;
@@ -599,16 +502,6 @@ static int kpt_for(int nb) {
 }
 
 template <class... Args>
-static void launch_col_update(int kpt, dim3 grid, hipStream_t s, Args... args) {
-  switch (kpt) {
-    case 1: hipLaunchKernelGGL(k_col_update<1>, grid, dim3(RB), 0, s, args...); break;
-    case 2: hipLaunchKernelGGL(k_col_update<2>, grid, dim3(RB), 0, s, args...); break;
-    case 4: hipLaunchKernelGGL(k_col_update<4>, grid, dim3(RB), 0, s, args...); break;
-    default: hipLaunchKernelGGL(k_col_update<8>, grid, dim3(RB), 0, s, args...); break;
-  }
-}
-
-template <class... Args>
 static void launch_wcol(int kpt, dim3 grid, hipStream_t s, Args... args) {
   switch (kpt) {
     case 1: hipLaunchKernelGGL(k_wcol<1>, grid, dim3(RB), 0, s, args...); break;
@@ -618,7 +511,7 @@ static void launch_wcol(int kpt, dim3 grid, hipStream_t s, Args... args) {
   }
 }
 
-// Per panel: k_col_update for its first column, then per column k_symv and
+// Per panel: k_col_norm for its first column, then per column k_symv and
 // k_wcol (which also updates the next column of the panel); then the
 // trailing update.
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s) {
@@ -636,8 +529,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       const int c = p + j;
       const int nblk = cdiv(n - c, RQ);            // row blocks from c
       if (j == 0)
-        launch_col_update(kpt, dim3(nblk), s, M, ld, n, p, c, j, w.W, ldw, w.KP, 0, w.part, w.TSP, w.ldtsp,
-                          ZB{nullptr, 0});
+        hipLaunchKernelGGL(k_col_norm, dim3(cdiv(n - c, 256)), dim3(256), 0, s, M, ld, n, c, w.part, ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
       const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
@@ -712,8 +604,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     const int nrow = L - c;
     const int nblk = cdiv(nrow, RQ);
     if (j == 0)
-      launch_col_update(kpt, dim3(nblk, 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.KP, 0, w.part, w.TSP, w.ldtsp,
-                        zb);
+      hipLaunchKernelGGL(k_col_norm, dim3(cdiv(nrow, 256), 1, act), dim3(256), 0, s, M, ld, L, c, w.part, zb);
     const int sz = L - c - 1;
     const int G = cdiv(sz, SG);
     const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);

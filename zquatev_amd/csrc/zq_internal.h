@@ -289,7 +289,6 @@ struct ReduceWork {
   double* TSP;    // their per-block partials: ceil(n/16) x ldtsp
   int ldtsp;      // 8 nb
   double* QF;     // per-tile partials of x^* M x: ceil(n/64) (ceil(n/64) + 1)
-  double* KP;     // ceil(n/16)
   double* part;   // ceil(n/16)
   double* scal;   // 16
 };
