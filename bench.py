@@ -33,7 +33,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n2", type=int, default=8192)
-    ap.add_argument("--nb", type=int, default=0, help="panel width (0: tuned table)")
+    ap.add_argument("--nb", type=int, default=0,
+                    help="panel width (0: autotuned over 32/64/128 before the warm-up; the built-in table "
+                         "with --profile-run)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -387,7 +389,16 @@ def main():
     ctx = zq.Context(0, panel_width=args.nb)
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
-    nb = args.nb if args.nb else zq.tuned_panel_width(n)
+    tune = None
+    if args.nb:
+        nb = args.nb
+    elif args.profile_run:
+        nb = ctx.panel_width(n)
+    else:
+        chosen, tms = ctx.autotune([n])
+        nb = chosen[0]
+        tune = {str(w): round(float(t), 3) for w, t in zip((32, 64, 128), tms[0]) if t >= 0}
+        print(f"autotune n={n}: reduction ms {tune} -> panel width {nb}", file=sys.stderr, flush=True)
 
     def step():
         Dt[:n, :].copy_(pristine)
@@ -509,7 +520,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "single solve, 2n=8192 complex128, A Hermitian / B antisymmetric U(-1,1), "
                                "eigenvalues + Kramers-paired eigenvectors",
-                   "n2": n2, "panel_width": nb, "seed": args.seed, "generator": "splitmix64",
+                   "n2": n2, "panel_width": nb, "panel_width_autotune_ms": tune, "seed": args.seed,
+                   "generator": "splitmix64",
                    "flop_count": "(160/3) n^3, n = 2n/2"},
         "event_ms_per_step": round(ev_ms, 3),
         "phases_ms": {k: round(v, 3) for k, v in phases.items()},

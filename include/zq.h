@@ -55,8 +55,24 @@ int zq_use_own_stream(zq_context* ctx);
  * table.  The reference hard-codes nb = 20 (zquatev.cc:64). */
 int zq_set_panel_width(zq_context* ctx, int nb);
 
-/* Panel width the tuned table picks for half-dimension n. */
+/* Panel width the built-in table picks for half-dimension n. */
 int zq_tuned_panel_width(int n);
+
+/* Panel-width autotuner (no counterpart in the reference, which hard-codes
+ * nb = 20 at zquatev.cc:64).  For each half-dimension sizes[k] it times the
+ * reduction, the one phase the panel width changes, with nb = 32, 64 and 128
+ * on a synthetic U(-1,1) matrix generated on the device, and records the
+ * fastest for the size bucket floor(log2 n) of this context; later solves of
+ * any n in that bucket use it unless zq_set_panel_width set an override.
+ * chosen[k] (may be NULL) receives the pick; ms[3k + t] (may be NULL) the
+ * best of two timed runs in milliseconds for candidate t, -1 where the
+ * candidate was skipped (wider than the matrix).  Synchronous; uses the
+ * context's workspace.  Returns 0, or -2 (sizes NULL or an entry < 1), -3
+ * (count < 0), or a ZQ_ERR_* code. */
+int zq_autotune(zq_context* ctx, const int* sizes, int count, int* chosen, double* ms);
+
+/* Panel width a solve of half-dimension n on this context uses now. */
+int zq_panel_width(zq_context* ctx, int n);
 
 /* Drop-in semantics of ts::zquatev (zquatev.h:54, zquatev.cc:42-100) on host
  * memory.  D: nld2 x n2 complex128 (the full buffer is used; only the left

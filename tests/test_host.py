@@ -69,6 +69,12 @@ def test_batched_entry_points_validate_without_device():
     assert g(None, 0, None, None, None, None, None) == zq.ERR_ARG
 
 
+def test_autotune_validates_without_device():
+    L = zq.lib()
+    assert L.zq_autotune(None, None, 0, None, None) == zq.ERR_ARG
+    assert L.zq_panel_width(None, 5) == 0
+
+
 def test_error_strings_and_tuning_table():
     assert zq.error_string(0) == "success"
     assert "odd" in zq.error_string(-1)

@@ -30,6 +30,8 @@ _SIGNATURES = {
     "zq_use_own_stream": (_c_int, [_vp]),
     "zq_set_panel_width": (_c_int, [_vp, _c_int]),
     "zq_tuned_panel_width": (_c_int, [_c_int]),
+    "zq_autotune": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
+    "zq_panel_width": (_c_int, [_vp, _c_int]),
     "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
     "zq_zquatev_ctx": (_c_int, [_vp, _c_int, _vp, _c_int, _vp]),
     "zq_zquatev_device": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int]),
@@ -187,6 +189,23 @@ class Context:
 
     def synchronize(self):
         _check(_fn("zq_synchronize")(self._h), "zq_synchronize")
+
+    def autotune(self, sizes):
+        """Time the reduction with panel widths 32, 64 and 128 for each
+        half-dimension in ``sizes`` and keep the fastest for that size's
+        power-of-two bucket on this context.  Returns ``(chosen, ms)``:
+        the pick per size and an array (len(sizes), 3) of the timings in ms
+        (-1 where a width was skipped as wider than the matrix)."""
+        sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
+        chosen = np.zeros(max(sz.size, 1), dtype=np.int32)
+        ms = np.zeros((max(sz.size, 1), 3), dtype=np.float64)
+        _check(_fn("zq_autotune")(self._h, sz.ctypes.data, int(sz.size), chosen.ctypes.data, ms.ctypes.data),
+               "zq_autotune")
+        return [int(v) for v in chosen[: sz.size]], ms[: sz.size]
+
+    def panel_width(self, n):
+        """Panel width a solve of half-dimension n on this context uses."""
+        return int(_fn("zq_panel_width")(self._h, int(n)))
 
     def zquatev(self, D, n2=None):
         """Host-memory solve on this context; same contract as :func:`zquatev`."""

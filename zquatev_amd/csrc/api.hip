@@ -36,6 +36,7 @@ struct zq_context {
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   int nb_override = 0;
+  int tuned_nb[32] = {};      // autotuned panel width per bucket floor(log2 n) (0: none)
   void* ws = nullptr;
   size_t ws_bytes = 0;
   void* dD = nullptr;       // staging copy of D for the host-memory entry points
@@ -283,8 +284,18 @@ int ensure(void*& ptr, size_t& have, size_t need) {
   return ZQ_OK;
 }
 
+int size_bucket(int n) {
+  int b = 0;
+  while (b < 30 && (2 << b) <= n) ++b;
+  return b;
+}
+
+// Panel width for half-dimension n: the caller's override, else the context's
+// autotuned pick for n's bucket, else the built-in table.
 int panel_width(const zq_context* ctx, int n) {
-  int nb = ctx->nb_override > 0 ? ctx->nb_override : zq_tuned_panel_width(n);
+  int nb = ctx->nb_override;
+  if (nb <= 0) nb = ctx->tuned_nb[size_bucket(n)];
+  if (nb <= 0) nb = zq_tuned_panel_width(n);
   if (nb > 128) nb = 128;
   if (nb < 1) nb = 1;
   return nb;
@@ -616,6 +627,67 @@ int zq_set_panel_width(zq_context* c, int nb) {
   if (!c) return -1;
   if (nb < 0 || nb > 128) return -2;
   c->nb_override = nb;
+  return ZQ_OK;
+}
+
+int zq_panel_width(zq_context* c, int n) {
+  if (!c || n < 1) return 0;
+  return panel_width(c, n);
+}
+
+// Times the reduction (the one phase the panel width changes) for each
+// candidate width on a synthetic U(-1,1) input generated on the device: one
+// warm-up and two timed runs each, the faster kept.  Candidates wider than
+// the matrix are skipped (they would repeat the previous one's schedule).
+int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double* ms) {
+  if (!c) return ZQ_ERR_ARG;
+  if (count < 0) return -3;
+  if (count > 0 && !sizes) return -2;
+  for (int k = 0; k < count; ++k)
+    if (sizes[k] < 1) return -2;
+  if (count == 0) return ZQ_OK;
+  int rc;
+  if ((rc = with_device(c))) return rc;
+  static const int cand[3] = {32, 64, 128};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return ZQ_ERR_HIP;
+  }
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  } ev{e0, e1};
+  hipStream_t s = c->stream;
+  for (int k = 0; k < count; ++k) {
+    const int n = sizes[k];
+    int best = 0;
+    double best_ms = 0.0;
+    for (int t = 0; t < 3; ++t) {
+      double tmin = -1.0;
+      if (t == 0 || cand[t - 1] < n - 1) {
+        Plan p;
+        if ((rc = make_plan(c, n, cand[t], false, &p))) return rc;
+        for (int rep = 0; rep < 3; ++rep) {
+          if ((rc = zq::gen_lower_uniform(n, 1001, p.M, n, s))) return rc;
+          ZQ_HIP_CHECK(hipEventRecord(e0, s));
+          if ((rc = zq::reduce(p.M, n, n, cand[t], p.d, p.e, p.rw, s))) return rc;
+          ZQ_HIP_CHECK(hipEventRecord(e1, s));
+          ZQ_HIP_CHECK(hipEventSynchronize(e1));
+          float f = 0.f;
+          ZQ_HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
+          if (rep > 0 && (tmin < 0.0 || f < tmin)) tmin = f;
+        }
+        if (best == 0 || tmin < best_ms) {
+          best = cand[t];
+          best_ms = tmin;
+        }
+      }
+      if (ms) ms[3 * k + t] = tmin;
+    }
+    c->tuned_nb[size_bucket(n)] = best;
+    if (chosen) chosen[k] = best;
+  }
   return ZQ_OK;
 }
 

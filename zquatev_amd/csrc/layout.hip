@@ -91,6 +91,34 @@ __global__ void k_scatter_info_b(int nprob, const BatchProb* __restrict__ P, con
   if (z < nprob) info_all[P[z].idx] = info[z];
 }
 
+// Synthetic lower triangle for timing runs: every real component of M_ij
+// (i > j) uniform in [-1, 1) from a SplitMix64 hash of (seed, i, j); the
+// diagonal is real (A Hermitian, B antisymmetric), as in the U(-1,1) workload.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u11(unsigned long long h) { return (double)(h >> 11) * 0x1.0p-52 - 1.0; }
+
+__global__ void __launch_bounds__(256) k_gen_lower(int n, unsigned long long seed, quat* __restrict__ M, int ld) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || i < j) return;
+  const unsigned long long base = splitmix64(seed) ^ (4ull * ((unsigned long long)j * (unsigned)n + (unsigned)i));
+  const bool dg = i == j;
+  const quat q{u11(splitmix64(base)), dg ? 0.0 : u11(splitmix64(base + 1)), dg ? 0.0 : u11(splitmix64(base + 2)),
+               dg ? 0.0 : u11(splitmix64(base + 3))};
+  qstore(M + i + (size_t)j * ld, q);
+}
+
+int gen_lower_uniform(int n, unsigned long long seed, quat* M, int ld, hipStream_t s) {
+  if (n <= 0) return ZQI_OK;
+  hipLaunchKernelGGL(k_gen_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, seed, M, ld);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
 int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* info_all, hipStream_t s) {
   if (nprob <= 0) return ZQI_OK;
   hipLaunchKernelGGL(k_scatter_info_b, dim3((nprob + 255) / 256), dim3(256), 0, s, nprob, P, info, info_all);

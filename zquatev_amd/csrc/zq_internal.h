@@ -404,6 +404,8 @@ int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* in
 int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s);
 int store_vectors(int n, const quat* X, int ldx, double* D, int ld2, hipStream_t s);
 int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
+// synthetic U(-1,1) lower triangle (real diagonal) for timing runs
+int gen_lower_uniform(int n, unsigned long long seed, quat* M, int ld, hipStream_t s);
 
 // gemm.hip: C = beta C + alpha opA(A) opB(B) for quaternion operands (FP64 MFMA).
 // An operand is the column concatenation [p0 | p1] split at ksplit along K;
