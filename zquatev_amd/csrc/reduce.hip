@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   __shared__ double red[8][4][SG];   // wave, component, row: conflict-free b64 accesses
   const int s0 = c + 1;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: its column loads are scalar
   const int b = blockIdx.x;
   const int Gt = (n - s0 + SG - 1) / SG;
   const int bt = b - Gt * (Gt + 1);
@@ -278,8 +279,10 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
 // including the new w_j(c+1) = p(s0) - K u1, are formed redundantly in every
 // block), store the column, and leave its tail-norm partials in part and the
 // per-16-row partials of W^* x, V^* x in TSP.
-template <int KPT>
-__global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n, int p, int c, int j,
+// PY, PQ: partial loads per thread per gather batch (quaternion row partials,
+// x^* M x partials); WPE: the waves per SIMD the register budget aims at.
+template <int KPT, int PY, int PQ, int WPE>
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) k_wcol(quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                              quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                              const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
                                              int nb, const double* __restrict__ scal, const double* __restrict__ QF,
@@ -292,6 +295,7 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
   }
   __shared__ quat a[NBMAX], b[NBMAX], cw[NBMAX], cv[NBMAX];
   __shared__ double red[16][8][RQ];   // k-group, (p | column update) component, row
+  __shared__ double red2[8][RQ];      // the same summed over the k-groups
   __shared__ double shr[RB / 64][10];
   __shared__ quat xs[RQ], us[RQ], ws[RQ];
   __shared__ double sh[RB / 64];
@@ -327,12 +331,11 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
   const int G = (n - s0 + SG - 1) / SG;
   // The matvec partials this thread sums, gathered in fixed-size batches of
   // predicated loads so that one wait covers them all (one batch up to
-  // trailing sizes of ~4k rows):
+  // trailing sizes of ~4k rows with the wide variant, ~1k with the light one):
   //   y(i):  slots k = q + 16 t of row i's list, Prow[J][i] for J <= li / SW
   //          (k < nJ), then Pcol[g][i] for g >= li / SG;
   //   x^* M x: QF[tid + RB t], t over the G (G + 1) tile partials;
   //   (M x)_{s0}: Prow[0][s0] and Pcol[g][s0] as slots tid + RB t of G + 1.
-  constexpr int PY = 10, PQ = 18;
   const int li = ic - s0;
   const int nJ = li / SW + 1, g0 = li / SG;
   const int cy = cdiv_d(nJ + G - g0 - q, 16);
@@ -422,17 +425,22 @@ __global__ void __launch_bounds__(RB) k_wcol(quat* __restrict__ M, int ld, int n
   red[q][0][r] = accp.ar; red[q][1][r] = accp.ai; red[q][2][r] = accp.br; red[q][3][r] = accp.bi;
   red[q][4][r] = accu.ar; red[q][5][r] = accu.ai; red[q][6][r] = accu.br; red[q][7][r] = accu.bi;
   __syncthreads();
+  // sums over the 16 k-groups, one (component, row) per thread
+  if (tid < 8 * RQ) {
+    const int rr = tid & (RQ - 1), comp = tid >> 4;
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k][comp][rr];
+    red2[comp][rr] = t;
+  }
+  __syncthreads();
   double ss = 0.0;
   if (q == 0) {
     quat xn = qzero(), ui = qzero(), wi = qzero();
     if (i < n) {
       const quat ms0 = (i == s0) ? quat{ms0raw.ar, 0.0, 0.0, 0.0} : ms0raw;   // Hermitian diagonal: real
-      quat pv = qmul(ms0, delta), su = qzero();
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        pv = qadd(pv, quat{red[k][0][r], red[k][1][r], red[k][2][r], red[k][3][r]});
-        su = qadd(su, quat{red[k][4][r], red[k][5][r], red[k][6][r], red[k][7][r]});
-      }
+      const quat pv = qadd(qmul(ms0, delta), quat{red2[0][r], red2[1][r], red2[2][r], red2[3][r]});
+      const quat su = quat{red2[4][r], red2[5][r], red2[6][r], red2[7][r]};
       ui = (i == s0) ? u1 : qscale(xi, gam);
       wi = qsub(pv, qscale(ui, K));
       qstore(W + i + (size_t)j * ldw, wi);
@@ -501,13 +509,23 @@ static int kpt_for(int nb) {
   return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
 }
 
+// k_wcol variants: light gathers (more blocks per CU: batches of small
+// problems) up to trailing size 1024, wide ones (one gather batch) above.
+template <int KPT, class... Args>
+static void launch_wcol_k(int sz, dim3 grid, hipStream_t s, Args... args) {
+  // register budgets that compile without spills: 3 waves per SIMD at KPT 1,
+  // 2 at KPT 2 and 4, 1 at KPT 8
+  constexpr int WPE = KPT == 1 ? 3 : (KPT == 8 ? 1 : 2);
+  if (sz <= 1024) hipLaunchKernelGGL((k_wcol<KPT, 4, 2, WPE>), grid, dim3(RB), 0, s, args...);
+  else hipLaunchKernelGGL((k_wcol<KPT, 10, 18, 1>), grid, dim3(RB), 0, s, args...);
+}
 template <class... Args>
-static void launch_wcol(int kpt, dim3 grid, hipStream_t s, Args... args) {
+static void launch_wcol(int kpt, int sz, dim3 grid, hipStream_t s, Args... args) {
   switch (kpt) {
-    case 1: hipLaunchKernelGGL(k_wcol<1>, grid, dim3(RB), 0, s, args...); break;
-    case 2: hipLaunchKernelGGL(k_wcol<2>, grid, dim3(RB), 0, s, args...); break;
-    case 4: hipLaunchKernelGGL(k_wcol<4>, grid, dim3(RB), 0, s, args...); break;
-    default: hipLaunchKernelGGL(k_wcol<8>, grid, dim3(RB), 0, s, args...); break;
+    case 1: launch_wcol_k<1>(sz, grid, s, args...); break;
+    case 2: launch_wcol_k<2>(sz, grid, s, args...); break;
+    case 4: launch_wcol_k<4>(sz, grid, s, args...); break;
+    default: launch_wcol_k<8>(sz, grid, s, args...); break;
   }
 }
 
@@ -539,7 +557,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
         launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
                  w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
       }
-      launch_wcol(kpt, dim3(cdiv(sz, RQ)), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal,
+      launch_wcol(kpt, sz, dim3(cdiv(sz, RQ)), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal,
                   w.QF, w.part, w.TSP, w.ldtsp, ZB{nullptr, 0});
     }
     // trailing update of rows/cols [p+wdt, n): M -= [V W] [W V]^*  (lower triangle)
@@ -610,7 +628,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
     hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
                        w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, w.QF, zb);
-    launch_wcol(kpt, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
+    launch_wcol(kpt, sz, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
                 w.scal, w.QF, w.part, w.TSP, w.ldtsp, zb);
     if (j == nb - 1 || c == L - 2) {
       const int P = c / nb;
