@@ -298,7 +298,6 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
   __shared__ double red2[8][RQ];      // the same summed over the k-groups
   __shared__ double shr[RB / 64][10];
   __shared__ quat xs[RQ], us[RQ], ws[RQ];
-  __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int s0 = c + 1;
@@ -460,8 +459,12 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
     ws[r] = wi;
   }
   if (!more) return;   // uniform
-  ss = block_sum(ss, sh);   // its barriers also publish xs, us, ws
-  if (tid == 0) part[blockIdx.x] = ss;
+  __syncthreads();   // publishes xs, us, ws
+  // the tail-norm terms live in lanes 0..15 of wave 0 (q == 0), zero elsewhere
+  if (wv == 0) {
+    ss = wave_sum_d(ss);
+    if (lane == 0) part[blockIdx.x] = ss;
+  }
   // tall-skinny partials of column c+1 over the panel columns 0..j (j + 1 of them):
   // this thread's columns l = q + 16 k against x(row r), summed over the 16 rows
   {
