@@ -15,11 +15,12 @@
 // level by level (all merges of a level in one launch):
 //   prep   : stable merge of the two sorted spectra, z = [last row of Q1;
 //            first row of Q2]/sqrt(2), rho = 2 beta, deflation scan (small
-//            |z| or close d handled by a Givens rotation), one workgroup.
-//   rotgat : apply the deflation rotations to Q's columns (row-parallel) and
-//            gather the kept and deflated columns into Qtmp.
-//   secular: one root of 1 + rho sum z_j^2/(d_j - lambda) per thread, by
-//            bisection on the offset from the nearer pole.
+//            |z| or close d handled by a Givens rotation), one workgroup,
+//            the scan on LDS copies up to PREP_LDS rows.
+//   rot    : apply the deflation rotations to Q's columns (row-parallel).
+//   gather : the kept and deflated columns into Qtmp (rows x column chunks).
+//   secular: one root of 1 + rho sum z_j^2/(d_j - lambda) per wavefront,
+//            safeguarded Newton on the offset from the nearer pole.
 //   zhat   : Gu-Eisenstat z from the computed roots, for orthogonality.
 //   buildU : U[:, i] = zhat / (d - lambda_i), normalised.
 //   order  : merge the roots with the deflated values (both sorted).
@@ -216,6 +217,11 @@ __device__ __forceinline__ int count_leq(const double* a, int len, double v) {
   return lo;
 }
 
+// Merges of up to PREP_LDS rows run the sequential deflation scan on copies
+// in LDS (a dependent chain of a few accesses per entry); larger ones in
+// global memory.
+constexpr int PREP_LDS = 4096;
+
 __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict__ descs,
                                                     const double* __restrict__ lam, const double* __restrict__ es,
                                                     const double* __restrict__ Z, int ldz, MergeBufs B) {
@@ -224,8 +230,11 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
   const int ns = hi - lo, n1 = mid - lo, n2 = hi - mid;
   const double beta = es[mid - 1];
   const double rho = 2.0 * beta;
-  double* dk = B.dk + lo;
-  double* zk = B.zk + lo;
+  __shared__ double s_dk[PREP_LDS], s_zk[PREP_LDS];
+  __shared__ int s_keep[PREP_LDS], s_dfl[PREP_LDS];
+  const bool inl = ns <= PREP_LDS;   // uniform
+  double* dk = inl ? s_dk : B.dk + lo;
+  double* zk = inl ? s_zk : B.zk + lo;
   int* src = B.src + lo;
   for (int t = threadIdx.x; t < ns; t += 256) {
     double v, zr;
@@ -260,11 +269,61 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
   dm = fmax(fmax(shd[0], shd[1]), fmax(shd[2], shd[3]));
   zm = fmax(fmax(shz[0], shz[1]), fmax(shz[2], shz[3]));
   const double tol = 8.0 * DEPS * fmax(dm, rho * zm);
-  int* keep = B.keep + lo;
-  int* dfl = B.dfl + lo;
+  int* keep = inl ? s_keep : B.keep + lo;
+  int* dfl = inl ? s_dfl : B.dfl + lo;
   double* rot = B.rot + 4 * (size_t)lo;
   __shared__ int sK;
-  if (threadIdx.x == 0) {
+  // Fast path.  The scan below changes d and z only where it rotates; if no
+  // consecutive pair of non-deflated entries passes the rotation test on the
+  // unrotated values, it rotates nowhere and reduces to two order-preserving
+  // compactions (kept: non-negligible z; deflated: the rest, already sorted).
+  // Each thread takes a contiguous chunk; scans over threads give its offsets.
+  __shared__ int scnt[256], slast[256];
+  const int C = (ns + 255) / 256;
+  const int j0 = min(ns, (int)threadIdx.x * C), j1 = min(ns, j0 + C);
+  int nk = 0, lastk = -1;
+  for (int j = j0; j < j1; ++j)
+    if (!(rho * fabs(zk[j]) <= tol)) { ++nk; lastk = j; }
+  scnt[threadIdx.x] = nk;
+  slast[threadIdx.x] = lastk;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {   // inclusive scans: sum of counts, max of last index
+    const int c2 = threadIdx.x >= off ? scnt[threadIdx.x - off] : 0;
+    const int l2 = threadIdx.x >= off ? slast[threadIdx.x - off] : -1;
+    __syncthreads();
+    scnt[threadIdx.x] += c2;
+    slast[threadIdx.x] = max(slast[threadIdx.x], l2);
+    __syncthreads();
+  }
+  const int kbase = scnt[threadIdx.x] - nk;
+  int pj0 = threadIdx.x > 0 ? slast[threadIdx.x - 1] : -1;
+  bool rotates = false;
+  for (int j = j0; j < j1; ++j) {
+    if (rho * fabs(zk[j]) <= tol) continue;
+    if (pj0 >= 0) {
+      double sr = zk[pj0], cr = zk[j];
+      const double tt = hypot(cr, sr);
+      const double t = dk[j] - dk[pj0];
+      cr /= tt;
+      sr = -sr / tt;
+      if (fabs(t * cr * sr) <= tol) rotates = true;
+    }
+    pj0 = j;
+  }
+  if (!__syncthreads_or(rotates)) {
+    int kk = kbase, dd = j0 - kbase;
+    for (int j = j0; j < j1; ++j) {
+      if (rho * fabs(zk[j]) <= tol) dfl[dd++] = j;
+      else keep[kk++] = j;
+    }
+    if (threadIdx.x == 0) {
+      const int K = scnt[255];
+      B.nrot[blockIdx.x] = 0;
+      B.kcnt[blockIdx.x] = K;
+      B.rho[blockIdx.x] = rho;
+      sK = K;
+    }
+  } else if (threadIdx.x == 0) {
     int K = 0, nd = 0, nr = 0, pj = -1;
     for (int j = 0; j < ns; ++j) {
       if (rho * fabs(zk[j]) <= tol) {
@@ -312,20 +371,23 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
     if (t < K) {
       B.dkk[lo + t] = dk[keep[t]];
       B.zkk[lo + t] = zk[keep[t]];
+      if (inl) B.keep[lo + t] = keep[t];
     } else {
       B.dval[lo + t - K] = dk[dfl[t - K]];
+      if (inl) B.dfl[lo + t - K] = dfl[t - K];
     }
   }
 }
 
-__global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restrict__ descs, double* __restrict__ Z,
-                                                      int ldz, MergeBufs B) {
+// The deflation rotations, applied in order to the merge's columns of Z
+// (one thread per row).
+__global__ void __launch_bounds__(256) k_merge_rot(const MergeDesc* __restrict__ descs, double* __restrict__ Z,
+                                                   int ldz, MergeBufs B) {
   const MergeDesc md = descs[blockIdx.y];
   const int lo = md.lo, ns = md.hi - md.lo, cb = md.lo - md.ob;
   const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= ns) return;
   const int nr = B.nrot[blockIdx.y];
-  const int K = B.kcnt[blockIdx.y];
+  if (r >= ns || nr == 0) return;
   const int* src = B.src + lo;
   const double* rot = B.rot + 4 * (size_t)lo;
   double* zr = Z + (lo + r);
@@ -337,11 +399,35 @@ __global__ void __launch_bounds__(256) k_merge_rotgat(const MergeDesc* __restric
     zr[cp] = c * x + s * y;
     zr[cj] = c * y - s * x;
   }
+}
+
+// Qtmp = the merge's block of Z with its columns in the order kept, then
+// deflated.  Grid: x = 256-row chunks, y = GATHER_COLS-column chunks, z = merge.
+constexpr int GATHER_COLS = 8;
+__global__ void __launch_bounds__(256) k_merge_gather(const MergeDesc* __restrict__ descs,
+                                                      const double* __restrict__ Z, int ldz, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.z];
+  const int lo = md.lo, ns = md.hi - md.lo, cb = md.lo - md.ob;
+  const int k0 = blockIdx.y * GATHER_COLS;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (k0 >= ns || r >= ns) return;
+  const int K = B.kcnt[blockIdx.z];
+  const int* src = B.src + lo;
   double* Q = B.Qtmp + (size_t)lo * B.lq;
-  for (int k = 0; k < ns; ++k) {
+  const double* zr = Z + (lo + r);
+  int col[GATHER_COLS];
+#pragma unroll
+  for (int u = 0; u < GATHER_COLS; ++u) {
+    const int k = min(k0 + u, ns - 1);
     const int sp = (k < K) ? B.keep[lo + k] : B.dfl[lo + k - K];
-    Q[r + (size_t)k * ns] = zr[(size_t)(cb + src[sp]) * ldz];
+    col[u] = cb + src[sp];
   }
+  double v[GATHER_COLS];
+#pragma unroll
+  for (int u = 0; u < GATHER_COLS; ++u) v[u] = zr[(size_t)col[u] * ldz];
+#pragma unroll
+  for (int u = 0; u < GATHER_COLS; ++u)
+    if (k0 + u < ns) Q[r + (size_t)(k0 + u) * ns] = v[u];
 }
 
 // One root of f(t) = 1 + rho sum_j z_j^2 / (delta_j - t) per wavefront, the 64
@@ -673,7 +759,9 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     for (size_t q = 0; q < cnt; ++q) nsmax = std::max(nsmax, flat[off + q].hi - flat[off + q].lo);
     const int nm = (int)cnt;
     hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, sw.lam, sw.es, sw.Z, ldz, B);
-    hipLaunchKernelGGL(k_merge_rotgat, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
+    hipLaunchKernelGGL(k_merge_rot, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
+    hipLaunchKernelGGL(k_merge_gather, dim3(cdiv(nsmax, 256), cdiv(nsmax, GATHER_COLS), nm), dim3(256), 0, s, dd,
+                       sw.Z, ldz, B);
     hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, B);
