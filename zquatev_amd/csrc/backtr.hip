@@ -109,17 +109,43 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, int wbt, const quat* __r
   const int wdt = min(wbt, n - 1 - p);
   const quat* G = Gall + (size_t)blk * wbt * wbt;
   quat* T = Tall + (size_t)blk * wbt * wbt;
-  for (int jj = 0; jj < wbt; ++jj) {
-    for (int r = threadIdx.x; r < wbt; r += 256) {
-      quat acc = qzero();
-      if (jj < wdt && r < jj) {
-#pragma unroll 4
-        for (int l = r; l < jj; ++l) qfma(acc, qload(T + r + (size_t)l * wbt), qload(G + l + (size_t)jj * wbt));
-        acc = qscale(acc, -1.0);
-      } else if (r == jj) {
-        acc = quat{1.0, 0.0, 0.0, 0.0};
+  // Columns in groups of TB: thread r (its row of T) first applies the
+  // columns l < J of the rows it owns (one read of T(r, l) serves the group's
+  // TB columns, whose G entries are staged in LDS), then runs the recurrence
+  // inside the group from registers.  Every entry sums l = r .. jj-1 in
+  // increasing order, as the column-by-column recurrence does.
+  constexpr int TB = 8;
+  __shared__ quat gs[TB][128];
+  const int r = threadIdx.x;
+  for (int J = 0; J < wbt; J += TB) {
+    for (int idx = threadIdx.x; idx < TB * wbt; idx += 256) {
+      const int c = idx / wbt, l = idx - c * wbt;
+      const int jj = J + c;
+      gs[c][l] = (jj < wdt && l < jj) ? qload(G + l + (size_t)jj * wbt) : qzero();
+    }
+    __syncthreads();
+    if (r < wbt) {
+      quat acc[TB];
+#pragma unroll
+      for (int c = 0; c < TB; ++c) acc[c] = qzero();
+#pragma unroll 2
+      for (int l = r; l < J; ++l) {
+        const quat t = qload(T + r + (size_t)l * wbt);
+#pragma unroll
+        for (int c = 0; c < TB; ++c) qfma(acc[c], t, gs[c][l]);
       }
-      qstore(T + r + (size_t)jj * wbt, acc);
+      quat tb[TB];
+#pragma unroll
+      for (int c = 0; c < TB; ++c) {
+        const int jj = J + c;
+        quat a = acc[c];
+#pragma unroll
+        for (int cc = 0; cc < c; ++cc)
+          if (r <= J + cc) qfma(a, tb[cc], gs[c][J + cc]);
+        if (jj < wdt && r < jj) tb[c] = qscale(a, -1.0);
+        else tb[c] = (r == jj) ? quat{1.0, 0.0, 0.0, 0.0} : qzero();
+        if (jj < wbt) qstore(T + r + (size_t)jj * wbt, tb[c]);
+      }
     }
     __syncthreads();
   }
