@@ -84,6 +84,25 @@ def test_batch_fock_workload(gpu_ctx):
     assert np.array_equal(eigs[3], e1) and np.array_equal(Xs[3], Xs1)
 
 
+def test_batch_wide_sizes(gpu_ctx):
+    """A member large enough for the wide column kernel (trailing sizes above
+    1024) next to small ones: each member bitwise equal to its solo solve."""
+    sizes = [1100, 40, 700]
+    _say(f"case: batch with wide column kernel, sizes {sizes}")
+    Ds = [gen.uniform_case(n, 6600 + n) for n in sizes]
+    rc, info, Xs, eigs = _batch(gpu_ctx, Ds)
+    assert rc == 0 and info == [0] * len(sizes)
+    for k, (n, D) in enumerate(zip(sizes, Ds)):
+        Xs1, e1 = _solo(gpu_ctx, D)
+        assert np.array_equal(eigs[k], e1), f"problem {k} (n={n}): eigenvalues differ from the solo solve"
+        assert np.array_equal(Xs[k], Xs1), f"problem {k} (n={n}): eigenvectors differ from the solo solve"
+        if n <= 700:
+            inv.assert_solution(gen.full_hermitian(D, n), Xs[k], eigs[k])
+        else:
+            H = gen.full_hermitian(D, n)
+            assert max(inv.eig_error(H, eigs[k])) <= inv.EIG_TOL
+
+
 def test_batch_values_only(gpu_ctx):
     sizes = [90, 1, 300, 17, 256]
     _say(f"case: values-only batch {sizes}")

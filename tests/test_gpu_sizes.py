@@ -115,6 +115,42 @@ def _device_checks(Dcols, n, eig, H, ncols=64):
     return res, orth, trace, struct, bool(torch.all(eig[1:] >= eig[:-1]).item())
 
 
+_EIG_1300 = {}
+
+
+@pytest.mark.parametrize("nb", [16, 32, 64, 128])
+def test_zquatev_wide_column_kernel(gpu_ctx, nb):
+    """n = 1300: trailing sizes above 1024 run the wide column kernel (one
+    instantiation per panel width class 16/32/64/128 columns), and the switch
+    to the light one (trailing size 1024, column 275) falls inside a panel
+    for every nb.  Device checks of the solve, and its eigenvalues against
+    LAPACK's on the full 2600 x 2600 H."""
+    import torch
+    n = 1300
+    _say(f"case: device solve 2n={2 * n} nb={nb}")
+    D = gen.uniform_case(n, 6100)
+    if "ref" not in _EIG_1300:
+        H = gen.full_hermitian(D, n)
+        _EIG_1300["ref"] = np.linalg.eigvalsh(H)[::2]      # each eigenvalue twice (Kramers)
+        _EIG_1300["norm"] = np.abs(_EIG_1300["ref"]).max()
+    Dcols = torch.from_numpy(np.ascontiguousarray(D.T)).cuda()
+    H = gen.full_hermitian_t(Dcols, n)
+    eig = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gpu_ctx.set_panel_width(nb)
+    try:
+        torch.cuda.synchronize()
+        rc = gpu_ctx.zquatev_device(2 * n, Dcols.data_ptr(), 2 * n, eig.data_ptr(), "V")
+        gpu_ctx.synchronize()
+    finally:
+        gpu_ctx.set_panel_width(0)
+    assert rc == 0
+    res, orth, trace, struct, ascending = _device_checks(Dcols, n, eig, H)
+    _say(f"  residual {res:.3e} orth {orth:.3e} trace {trace:.3e}")
+    assert res <= inv.RES_TOL and orth <= inv.ORTH_TOL and trace <= 1e-13 and struct and ascending
+    err = np.abs(eig.cpu().numpy() - _EIG_1300["ref"]).max()
+    assert err <= inv.EIG_TOL * _EIG_1300["norm"], err
+
+
 def test_zquatev_2n32768_large_solve(gpu_ctx):
     """The large single solve at full size (2n = 32768, ~17 GB matrix resident
     in HBM, uniform entries with diag(A) + 10): trace identity, residual and

@@ -147,7 +147,13 @@ using zq::quat;
 constexpr int kNSplit = 8;
 
 // reflectors per back-transform block (a multiple of the panel width)
-inline int backtr_width(int n) { return n > 1024 ? 128 : 64; }
+// Reflectors per compact-WY block of the back-transform: one width for every
+// n, so that a batch member (laid out for its group's largest n) applies the
+// same blocks as its solo solve; 128 measured as fast as 64 for small n.
+inline int backtr_width(int n) {
+  (void)n;
+  return 128;
+}
 
 struct Plan {
   quat* M;

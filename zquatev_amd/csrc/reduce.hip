@@ -145,11 +145,33 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   if (bt >= 0) {
     // ---- finishing block: reflector, scalars, tall-skinny products
     const bool has_ts = bt < NT;
+    // every load of the block is issued before the first wait: the row-s0
+    // entries of W and V (for the u_1 term of TS), the diagonal entries, and
+    // the partials in batches of 16 per lane (one batch up to m = 2048)
+    const int e0 = 64 * bt + 4 * (tid & 15);
+    const int lts = e0 >> 3, side = (e0 >> 2) & 1;
+    const bool fin = has_ts && tid < 16 && lts < j;
+    quat rowv = qzero();
+    if (fin) rowv = side == 0 ? qload(W + s0 + (size_t)lts * ldw) : qload(M + s0 + (size_t)(p + lts) * ld);
+    double mss = 0.0, mcc = 0.0;
+    if (bt == 0 && tid == 0) {
+      mss = M[s0 + (size_t)s0 * ld].ar;
+      mcc = M[c + (size_t)c * ld].ar;
+    }
     double tsacc = 0.0;
     if (has_ts) {
       const int eidx = 64 * bt + lane;
-#pragma unroll 8
-      for (int k = w; k < nparts; k += 8) tsacc += TSP[(size_t)k * ldtsp + eidx];
+      constexpr int TB = 16;
+      for (int k0 = w; k0 < nparts; k0 += 8 * TB) {
+        double v[TB];
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+          const int k = k0 + 8 * u;
+          v[u] = k < nparts ? TSP[(size_t)k * ldtsp + eidx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) tsacc += v[u];
+      }
     }
     const quat x1 = qload(M + s0 + (size_t)c * ld);
     double tail = 0.0;
@@ -162,32 +184,27 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       scal[1] = R.u1.ar; scal[2] = R.u1.ai; scal[3] = R.u1.br; scal[4] = R.u1.bi;
       scal[5] = x1.ar; scal[6] = x1.ai; scal[7] = x1.br; scal[8] = x1.bi;
       // the trailing diagonal M[s0, s0] before k_wcol's update of column s0 overwrites it
-      scal[9] = M[s0 + (size_t)s0 * ld].ar;
-      d[c] = M[c + (size_t)c * ld].ar;
+      scal[9] = mss;
+      d[c] = mcc;
       qstore(e + c, alpha);
     }
     if (has_ts) {
       __shared__ double tsr[8][64];
       tsr[w][lane] = tsacc;
       __syncthreads();
-      if (tid < 16) {
-        // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
-        const int e0 = 64 * bt + 4 * tid;
-        const int l = e0 >> 3, side = (e0 >> 2) & 1;
-        if (l < j) {
-          double sc[4];
+      // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
+      if (fin) {
+        double sc[4];
 #pragma unroll
-          for (int comp = 0; comp < 4; ++comp) {
-            double t = 0.0;
+        for (int comp = 0; comp < 4; ++comp) {
+          double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
-            sc[comp] = t;
-          }
-          const quat rowv = side == 0 ? qload(W + s0 + (size_t)l * ldw) : qload(M + s0 + (size_t)(p + l) * ld);
-          quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
-          qfma_cl(t, rowv, R.u1);
-          qstore(TS + (size_t)side * nb + l, t);
+          for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
+          sc[comp] = t;
         }
+        quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
+        qfma_cl(t, rowv, R.u1);
+        qstore(TS + (size_t)side * nb + lts, t);
       }
     }
     return;
