@@ -309,7 +309,8 @@ def run_serving(args):
                     for c in srv.poll():
                         cdone[c["ticket"]] = c
             restore(s_)
-            tk = srv.submit(sizes[s_], work[s_].data_ptr(), sizes[s_], eigs[s_].data_ptr(), "V" if vec[r] else "N")
+            tk = srv.submit(sizes[s_], work[s_].data_ptr(), sizes[s_], eigs[s_].data_ptr(),
+                            "V" if vec[r % count] else "N")
             last_cap[s_] = tk
             csub[tk] = srv.clock_us()
         srv.drain()

@@ -1,6 +1,6 @@
 // Real symmetric tridiagonal eigensolver on the device: Cuppen's divide and
-// conquer with Gu-Eisenstat eigenvectors, and Sturm bisection for the
-// eigenvalues-only path.
+// conquer with Gu-Eisenstat eigenvectors, and Sturm-count multisection for
+// the eigenvalues-only path.
 //
 // Replaces the tridiagonal step of the reference, zhbev_("V","L",n,1,...)
 // (zquatev.cc:74-84; MKL: zhbtrd phase scaling + zsteqr QL/QR).  The phase
@@ -631,15 +631,32 @@ __global__ void k_dc_unscale(const int* __restrict__ probs, const double* __rest
 }
 
 // --------------------------------------------------------------------------
-// eigenvalues only: Sturm-count bisection, one eigenvalue per thread, on the
+// eigenvalues only: Sturm-count multisection, one eigenvalue per wavefront, on the
 // scaled tridiagonal.
 // --------------------------------------------------------------------------
+// The recurrence is a chain of divisions; its operands are loaded eight at a
+// time ahead of the chain so that the loads do not sit on it.
 __device__ __forceinline__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++cnt;
-  for (int i = 1; i < n; ++i) {
+  int i = 1;
+  for (; i + 8 <= n; i += 8) {
+    double dd[8], ee[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dd[u] = d[i + u];
+      ee[u] = e2[i + u - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      q = (dd[u] - x) - ee[u] / q;
+      if (fabs(q) < pivmin) q = -pivmin;
+      if (q < 0.0) ++cnt;
+    }
+  }
+  for (; i < n; ++i) {
     q = (d[i] - x) - e2[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++cnt;
@@ -647,24 +664,51 @@ __device__ __forceinline__ int sturm_count(int n, const double* d, const double*
   return cnt;
 }
 
-__global__ void __launch_bounds__(256) k_bisect(const int* __restrict__ probs, const double* __restrict__ d,
-                                                const double* __restrict__ e2, double gl, double gu, double pivmin,
-                                                const double* __restrict__ scal, double* __restrict__ w) {
+// Eigenvalues of problem blockIdx.y by multisection, MS_LANES lanes per
+// eigenvalue (64 / MS_LANES per wavefront): each step the group's lanes
+// evaluate Sturm counts at the points a + (b - a) (l + 1) / (MS_LANES + 1), and
+// the first lane whose count exceeds k brackets the eigenvalue with its left
+// neighbour, so that count(a) <= k < count(b) holds after every step
+// (whatever the rounding of the counts) and the bracket shrinks
+// (MS_LANES + 1)x per step.  More lanes mean fewer steps of the chain of
+// divisions each lane runs, but more count work in total: 64 lanes are
+// faster while the eigenvalues do not fill the device (n up to ~2.5k), 16
+// above.
+template <int MS_LANES>
+__global__ void __launch_bounds__(256) k_multisect(const int* __restrict__ probs, const double* __restrict__ d,
+                                                   const double* __restrict__ e2, double gl, double gu, double pivmin,
+                                                   const double* __restrict__ scal, double* __restrict__ w) {
   const int o = probs[2 * blockIdx.y], n = probs[2 * blockIdx.y + 1];
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / MS_LANES, gl0 = grp * MS_LANES, li = lane - gl0;
+  const int k = blockIdx.x * (256 / MS_LANES) + threadIdx.x / MS_LANES;
+  const int wave_k0 = blockIdx.x * (256 / MS_LANES) + (threadIdx.x >> 6) * (64 / MS_LANES);
+  if (wave_k0 >= n) return;   // whole wavefront
   d += o; e2 += o; w += o;
-  double a = gl, b = gu;   // count(a) <= k < count(b)
-  // interval width down to a few ulps of the eigenvalue, or of the matrix
+  double a = gl, b = gu;
+  bool live = k < n;          // this group still narrowing (uniform within the group)
+  // bracket width down to a few ulps of the eigenvalue, or of the matrix
   // norm (1 after scaling) for eigenvalues near zero: an absolute error of
   // eps ||T|| is what any backward-stable eigensolver commits
-  for (int it = 0; it < 200; ++it) {
-    const double m = 0.5 * (a + b);
-    if (!(m > a && m < b)) break;
-    if (b - a <= 2.0 * DEPS * fmax(fmax(fabs(a), fabs(b)), 1.0)) break;
-    if (sturm_count(n, d, e2, m, pivmin) > k) b = m; else a = m;
+  for (int it = 0; it < 64; ++it) {
+    if (live && b - a <= 2.0 * DEPS * fmax(fmax(fabs(a), fabs(b)), 1.0)) live = false;
+    if (!__any(live)) break;
+    const double x = a + (b - a) * ((double)(li + 1) * (1.0 / (MS_LANES + 1)));
+    const bool up = live && sturm_count(n, d, e2, x, pivmin) > k;
+    const unsigned long long all = __ballot(up);
+    const unsigned long long grpmask = MS_LANES == 64 ? all : ((all >> gl0) & ((1ull << (MS_LANES % 64)) - 1));
+    const int f = grpmask ? __ffsll((long long)grpmask) - 1 : MS_LANES;
+    const double xb = __shfl(x, gl0 + (f < MS_LANES ? f : MS_LANES - 1));
+    const double xl = __shfl(x, gl0 + (f > 0 ? f - 1 : 0));
+    if (live) {
+      const double na = f == 0 ? a : xl;
+      const double nbd = f < MS_LANES ? xb : b;
+      if (na == a && nbd == b) live = false;   // the points no longer split the bracket
+      a = na;
+      b = nbd;
+    }
   }
-  w[k] = 0.5 * (a + b) * scal[blockIdx.y];
+  if (li == 0 && k < n) w[k] = 0.5 * (a + b) * scal[blockIdx.y];
 }
 
 __global__ void k_square(int n, const double* __restrict__ es, double* __restrict__ e2) {
@@ -738,8 +782,12 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     // Gershgorin interval of every scaled matrix lies in [-3, 3]
     hipLaunchKernelGGL(k_square, dim3(cdiv(ntot, 256)), dim3(256), 0, s, ntot, sw.es, sw.e2);
     const double pivmin = DBL_MIN * 4.0;
-    hipLaunchKernelGGL(k_bisect, dim3(cdiv(nmax, 256), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
-                       -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
+    if (nmax <= 2560)
+      hipLaunchKernelGGL(k_multisect<64>, dim3(cdiv(nmax, 4), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
+                         -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
+    else
+      hipLaunchKernelGGL(k_multisect<16>, dim3(cdiv(nmax, 16), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
+                         -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
     return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
   }
   const int nleaf = (int)leaves.size() / 4;
