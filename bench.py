@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-values-only", action="store_true", help="skip the eigenvalues-only (jobz='N') summary")
     ap.add_argument("--workload", choices=("single", "batched", "serving"), default="single",
                     help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only; "
                          "serving: the Poisson request stream through the server and the plain loop")
@@ -494,6 +495,26 @@ def main():
                  "ok": bool(res <= 1e-13 and orth <= 1e-13 and trace <= 1e-13)}
         del H
 
+    # eigenvalues only (jobz = 'N'): the same input, timed on its own count (64/3) n^3
+    vals = None
+    if not args.no_values_only:
+        eig_v = eig.clone()
+        eig_n = torch.zeros_like(eig)
+        tv = []
+        for r in range(3):
+            Dt[:n, :].copy_(pristine)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            rc = ctx.zquatev_device(n2, Dt.data_ptr(), n2, eig_n.data_ptr(), "N")
+            torch.cuda.synchronize()
+            if rc != 0:
+                raise RuntimeError(f"zq_zquatev_device(jobz='N') failed ({rc})")
+            tv.append(time.perf_counter() - t1)
+        tmin = min(tv[1:])
+        vals = {"ms": round(1000.0 * tmin, 3), "gflops_on_64_3_n3": round(gen.flops_values(n) / tmin / 1e9, 1),
+                "phases_ms": {k: round(v, 3) for k, v in ctx.last_timings().items()},
+                "max_abs_diff_vs_vectors": float((eig_n - eig_v).abs().max().item())}
+
     # instrumented pass: per-kernel-class HIP event times on the library's stream
     ctx.set_instrument(True)
     step()
@@ -562,6 +583,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "batched": batched,
+        "values_only": vals,
     }
     print(json.dumps(line))
     return 0
