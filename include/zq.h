@@ -21,6 +21,8 @@
 #ifndef ZQ_H
 #define ZQ_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -29,6 +31,7 @@ extern "C" {
 #define ZQ_ERR_HIP (-1001)    /* a HIP runtime call failed */
 #define ZQ_ERR_NOMEM (-1002)  /* device memory exhausted */
 #define ZQ_ERR_ARG (-1003)    /* unsupported internal configuration */
+#define ZQ_ERR_GRAPH (-1004)  /* graph capture: a step that cannot be captured, or resources held by graphs */
 
 typedef struct zq_context zq_context;
 
@@ -73,6 +76,26 @@ int zq_autotune(zq_context* ctx, const int* sizes, int count, int* chosen, doubl
 
 /* Panel width a solve of half-dimension n on this context uses now. */
 int zq_panel_width(zq_context* ctx, int n);
+
+/* HIP graph capture (no counterpart in the reference, a host library with no
+ * streams).  zq_zquatev_device and zq_zquatev_batched_device (with info =
+ * NULL) may be called while the context's stream is being captured
+ * (hipStreamBeginCapture on the stream set with zq_set_stream): the solve is
+ * recorded, nothing runs, and replays of the graph solve the same buffers
+ * again (the caller refreshes the inputs in place between replays).  For
+ * that, the call must find its workspace already allocated (run the same
+ * call once outside capture first), descriptor uploads are recorded from a
+ * pinned pool the context owns (8 MiB at zq_create), and from the first
+ * capture on the workspace is held: a later call needing more workspace
+ * returns ZQ_ERR_GRAPH until zq_graph_release.  Calls that must wait
+ * (info != NULL, zq_autotune) return ZQ_ERR_GRAPH during a capture.
+ *
+ * zq_graph_reserve: size the pinned upload pool to `bytes` (only while no
+ * captured graph holds it).  zq_graph_release: the caller has destroyed
+ * every graph captured on this context; frees the pool's contents and the
+ * workspace for reuse and reallocation. */
+int zq_graph_reserve(zq_context* ctx, size_t bytes);
+int zq_graph_release(zq_context* ctx);
 
 /* Drop-in semantics of ts::zquatev (zquatev.h:54, zquatev.cc:42-100) on host
  * memory.  D: nld2 x n2 complex128 (the full buffer is used; only the left

@@ -75,6 +75,13 @@ def test_autotune_validates_without_device():
     assert L.zq_panel_width(None, 5) == 0
 
 
+def test_graph_calls_validate_without_device():
+    L = zq.lib()
+    assert L.zq_graph_reserve(None, 1 << 20) == zq.ERR_ARG
+    assert L.zq_graph_release(None) == zq.ERR_ARG
+    assert "graph" in zq.error_string(zq.ERR_GRAPH)
+
+
 def test_error_strings_and_tuning_table():
     assert zq.error_string(0) == "success"
     assert "odd" in zq.error_string(-1)

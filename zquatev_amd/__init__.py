@@ -32,6 +32,8 @@ _SIGNATURES = {
     "zq_tuned_panel_width": (_c_int, [_c_int]),
     "zq_autotune": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
     "zq_panel_width": (_c_int, [_vp, _c_int]),
+    "zq_graph_reserve": (_c_int, [_vp, ctypes.c_size_t]),
+    "zq_graph_release": (_c_int, [_vp]),
     "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
     "zq_zquatev_ctx": (_c_int, [_vp, _c_int, _vp, _c_int, _vp]),
     "zq_zquatev_device": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _c_int]),
@@ -96,6 +98,7 @@ def _fn(name):
 ERR_HIP = -1001
 ERR_NOMEM = -1002
 ERR_ARG = -1003
+ERR_GRAPH = -1004
 
 
 class ZQError(RuntimeError):
@@ -202,6 +205,16 @@ class Context:
         _check(_fn("zq_autotune")(self._h, sz.ctypes.data, int(sz.size), chosen.ctypes.data, ms.ctypes.data),
                "zq_autotune")
         return [int(v) for v in chosen[: sz.size]], ms[: sz.size]
+
+    def graph_reserve(self, nbytes):
+        """Size the pinned pool that uploads recorded into captured graphs
+        come from (only while no captured graph holds it)."""
+        _check(_fn("zq_graph_reserve")(self._h, int(nbytes)), "zq_graph_reserve")
+
+    def graph_release(self):
+        """Every graph captured on this context has been destroyed: free the
+        pool's contents and let the workspace be reallocated again."""
+        _check(_fn("zq_graph_release")(self._h), "zq_graph_release")
 
     def panel_width(self, n):
         """Panel width a solve of half-dimension n on this context uses."""

@@ -37,6 +37,7 @@ struct zq_context {
   hipStream_t own_stream = nullptr;
   int nb_override = 0;
   int tuned_nb[32] = {};      // autotuned panel width per bucket floor(log2 n) (0: none)
+  bool ws_locked = false;     // captured graphs reference ws / binfo: no reallocation until released
   void* ws = nullptr;
   size_t ws_bytes = 0;
   void* dD = nullptr;       // staging copy of D for the host-memory entry points
@@ -88,6 +89,7 @@ void prof_take(hipEvent_t* e0, hipEvent_t* e1) {
   g_pending = -1;
 }
 int stage_begin(Staging& st) {
+  if (st.capture) return ZQI_OK;
   st.slot ^= 1;
   st.used = 0;
   if (st.done[st.slot] && hipEventSynchronize(st.done[st.slot]) != hipSuccess) return ZQI_ERR_HIP;
@@ -96,6 +98,14 @@ int stage_begin(Staging& st) {
 
 int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStream_t s) {
   if (bytes == 0) return ZQI_OK;
+  if (st.capture) {
+    const size_t off = (st.keep_used + 255) & ~size_t(255);
+    if (!st.keep || off + bytes > st.keep_cap) return ZQI_ERR_GRAPH;
+    std::memcpy(st.keep + off, src, bytes);
+    if (hipMemcpyAsync(dst, st.keep + off, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZQI_ERR_HIP;
+    st.keep_used = off + bytes;
+    return ZQI_OK;
+  }
   const int k = st.slot;
   size_t off = (st.used + 255) & ~size_t(255);
   if (off + bytes > st.cap[k]) {
@@ -123,6 +133,7 @@ int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStrea
 }
 
 int stage_end(Staging& st, hipStream_t s) {
+  if (st.capture) return ZQI_OK;
   const int k = st.slot;
   if (!st.done[k] && hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming) != hipSuccess) return ZQI_ERR_HIP;
   return hipEventRecord(st.done[k], s) == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
@@ -137,6 +148,9 @@ void stage_free(Staging& st) {
     st.done[k] = nullptr;
     st.cap[k] = 0;
   }
+  if (st.keep) (void)hipHostFree(st.keep);
+  st.keep = nullptr;
+  st.keep_cap = st.keep_used = 0;
 }
 }  // namespace zq
 
@@ -145,6 +159,18 @@ namespace {
 using zq::quat;
 
 constexpr int kNSplit = 8;
+constexpr size_t kGraphPool = size_t(8) << 20;   // pinned upload pool for captured graphs
+
+// Marks the context as capturing while the stream is being captured into a
+// graph (for the duration of one entry-point call); ZQ_ERR_GRAPH when the
+// capture cannot be served.
+struct CaptureScope {
+  zq_context* c;
+  int rc = ZQ_OK;
+  bool active = false;
+  explicit CaptureScope(zq_context* ctx);
+  ~CaptureScope();
+};
 
 // reflectors per back-transform block (a multiple of the panel width)
 // Reflectors per compact-WY block of the back-transform: one width for every
@@ -314,9 +340,16 @@ int panel_width(const zq_context* ctx, int n) {
   return nb;
 }
 
+// ensure() for buffers that captured graphs may reference: while they are
+// held (or during a capture) a buffer can be used but not reallocated.
+int ensure_held(const zq_context* ctx, void*& ptr, size_t& have, size_t need) {
+  if ((ctx->ws_locked || ctx->stage.capture) && (!ptr || have < need)) return ZQ_ERR_GRAPH;
+  return ensure(ptr, have, need);
+}
+
 int make_plan(zq_context* ctx, int n, int nb, bool vectors, Plan* p) {
   const size_t need = layout_plan(n, nb, vectors, nullptr, p);
-  int rc = ensure(ctx->ws, ctx->ws_bytes, need);
+  int rc = ensure_held(ctx, ctx->ws, ctx->ws_bytes, need);
   if (rc) return rc;
   layout_plan(n, nb, vectors, static_cast<char*>(ctx->ws), p);
   return ZQ_OK;
@@ -329,20 +362,22 @@ int with_device(zq_context* ctx) {
 // The device solve: dD (ld2 complex) -> dD, deig.
 int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool vectors) {
   hipStream_t s = ctx->stream;
+  const bool cap = ctx->stage.capture;   // no timing events or instrumentation inside a graph
   ctx->prof.used = 0;
-  zq::g_prof = &ctx->prof;
+  zq::g_prof = cap ? nullptr : &ctx->prof;
   struct Reset { ~Reset() { zq::g_prof = nullptr; } } reset_prof;
+  auto mark = [&](int k) { return cap ? hipSuccess : hipEventRecord(ctx->ev[k], s); };
   const int nb = panel_width(ctx, n);
   Plan p;
   int rc = make_plan(ctx, n, nb, vectors, &p);
   if (rc) return rc;
-  ZQ_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
+  ZQ_HIP_CHECK(mark(0));
   if ((rc = zq::load_left_half(n, dD, ld2, p.M, p.ldm, s))) return rc;
-  ZQ_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
+  ZQ_HIP_CHECK(mark(1));
   if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
   if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
-  ZQ_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
-  if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
+  ZQ_HIP_CHECK(mark(2));
+  if (vectors && (rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
   ctx->info_dev = vectors ? p.sw.info : nullptr;
   if ((rc = zq::stage_begin(ctx->stage))) return rc;
   struct EndStage {
@@ -353,18 +388,18 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   const int zero = 0;
   p.sw.stage = &ctx->stage;
   if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
-  ZQ_HIP_CHECK(hipEventRecord(ctx->ev[3], s));
+  ZQ_HIP_CHECK(mark(3));
   if (vectors) {
     if ((rc = zq::backtransform_batched(1, &n, zq::ZB{nullptr, 0}, nullptr, p.M, p.ldm, p.e, p.sw.Z, n, p.X, n, p.bw,
                                         p.btasks, p.bsums, ctx->stage, s)))
       return rc;
-    ZQ_HIP_CHECK(hipEventRecord(ctx->ev[4], s));
+    ZQ_HIP_CHECK(mark(4));
     if ((rc = zq::store_vectors(n, p.X, n, dD, ld2, s))) return rc;
   } else {
-    ZQ_HIP_CHECK(hipEventRecord(ctx->ev[4], s));
+    ZQ_HIP_CHECK(mark(4));
   }
-  ZQ_HIP_CHECK(hipEventRecord(ctx->ev[5], s));
-  ctx->have_times = true;
+  ZQ_HIP_CHECK(mark(5));
+  ctx->have_times = !cap;
   return ZQ_OK;
 }
 
@@ -445,7 +480,7 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
     return rc;
   if ((rc = zq::gather_tridiag_batched(nprob, L, bp.probs, bp.slab.d, bp.slab.e, zb, bp.dall, bp.eall, s))) return rc;
   bp.sw.stage = &st;
-  if (vectors) ZQ_HIP_CHECK(hipMemsetAsync(bp.sw.info, 0, nprob * sizeof(int), s));
+  if (vectors && (rc = zq::fill_zero(bp.sw.info, nprob * sizeof(int), s))) return rc;
   if ((rc = zq::stedc(nprob, zrow.data(), nh.data(), ntot, L, bp.dall, bp.eall, bp.wall, bp.sw, vectors, s)))
     return rc;
   if ((rc = zq::scatter_eig_batched(nprob, L, bp.probs, bp.wall, s))) return rc;
@@ -508,14 +543,15 @@ int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, c
     return ZQ_OK;
   }
   const size_t need = G.need;
-  int rc = ensure(ctx->ws, ctx->ws_bytes, need);
+  int rc = ensure_held(ctx, ctx->ws, ctx->ws_bytes, need);
   if (rc) return rc;
-  if ((rc = ensure(reinterpret_cast<void*&>(ctx->binfo), ctx->binfo_bytes, (size_t)count * sizeof(int) + 256)))
+  if ((rc = ensure_held(ctx, reinterpret_cast<void*&>(ctx->binfo), ctx->binfo_bytes,
+                        (size_t)count * sizeof(int) + 256)))
     return rc;
   ctx->have_times = false;
   ctx->info_dev = nullptr;
   hipStream_t s = ctx->stream;
-  ZQ_HIP_CHECK(hipMemsetAsync(ctx->binfo, 0, (size_t)count * sizeof(int), s));
+  if ((rc = zq::fill_zero(ctx->binfo, (size_t)count * sizeof(int), s))) return rc;
   if ((rc = zq::stage_begin(ctx->stage))) return rc;
   for (size_t g = 0; g < G.probs.size() && !rc; ++g) rc = solve_group(ctx, G.probs[g], G.nb[g], vectors);
   const int rc2 = zq::stage_end(ctx->stage, s);
@@ -572,6 +608,10 @@ const char* zq_error_string(int code) {
     case ZQ_ERR_HIP: return "HIP runtime error";
     case ZQ_ERR_NOMEM: return "device memory exhausted";
     case ZQ_ERR_ARG: return "unsupported internal configuration";
+    case ZQ_ERR_GRAPH:
+      return "graph capture: this call cannot be captured, its workspace is not allocated yet (run it once "
+             "outside capture), or the upload pool / workspace is held by captured graphs (zq_graph_reserve, "
+             "zq_graph_release)";
     case -1: return "argument 1 (n2) is negative or odd";
     case -2: return "argument 2 (D) is null";
     case -3: return "argument 3 (nld2) is smaller than n2";
@@ -605,7 +645,39 @@ int zq_create(zq_context** out, int device) {
       return ZQ_ERR_HIP;
     }
   }
+  // pinned pool for uploads recorded into captured graphs (allocated here:
+  // nothing may be allocated during a capture)
+  void* h = nullptr;
+  if (hipHostMalloc(&h, kGraphPool, hipHostMallocDefault) == hipSuccess) {
+    c->stage.keep = static_cast<char*>(h);
+    c->stage.keep_cap = kGraphPool;
+  } else {
+    (void)hipGetLastError();
+  }
   *out = c;
+  return ZQ_OK;
+}
+
+int zq_graph_reserve(zq_context* c, size_t bytes) {
+  if (!c) return ZQ_ERR_ARG;
+  if (c->stage.capture || c->stage.keep_used > 0) return ZQ_ERR_GRAPH;
+  if (bytes <= c->stage.keep_cap) return ZQ_OK;
+  void* h = nullptr;
+  if (hipHostMalloc(&h, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return ZQ_ERR_NOMEM;
+  }
+  if (c->stage.keep) (void)hipHostFree(c->stage.keep);
+  c->stage.keep = static_cast<char*>(h);
+  c->stage.keep_cap = bytes;
+  return ZQ_OK;
+}
+
+int zq_graph_release(zq_context* c) {
+  if (!c) return ZQ_ERR_ARG;
+  if (c->stage.capture) return ZQ_ERR_GRAPH;
+  c->stage.keep_used = 0;
+  c->ws_locked = false;
   return ZQ_OK;
 }
 
@@ -645,6 +717,26 @@ int zq_set_panel_width(zq_context* c, int nb) {
   return ZQ_OK;
 }
 
+CaptureScope::CaptureScope(zq_context* ctx) : c(ctx) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (c->stream && hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = ZQ_ERR_HIP;
+    return;
+  }
+  if (cs == hipStreamCaptureStatusNone) return;
+  if (cs != hipStreamCaptureStatusActive || !c->stage.keep) {
+    rc = ZQ_ERR_GRAPH;
+    return;
+  }
+  active = true;
+  c->stage.capture = true;
+  c->ws_locked = true;
+}
+CaptureScope::~CaptureScope() {
+  if (active) c->stage.capture = false;
+}
+
 int zq_panel_width(zq_context* c, int n) {
   if (!c || n < 1) return 0;
   return panel_width(c, n);
@@ -663,6 +755,11 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
   if (count == 0) return ZQ_OK;
   int rc;
   if ((rc = with_device(c))) return rc;
+  {
+    CaptureScope cap(c);
+    if (cap.rc) return cap.rc;
+    if (cap.active) return ZQ_ERR_GRAPH;   // it waits on its timing events
+  }
   static const int cand[3] = {32, 64, 128};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -713,6 +810,8 @@ int zq_zquatev_device(zq_context* c, int n2, double* dD, int nld2, double* deig,
   if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -5;
   if (n2 == 0) return ZQ_OK;
   if ((rc = with_device(c))) return rc;
+  CaptureScope cap(c);
+  if (cap.rc) return cap.rc;
   return solve_device(c, n2 / 2, dD, nld2, deig, jobz == 'V' || jobz == 'v');
 }
 
@@ -745,6 +844,9 @@ int zq_zquatev_batched_device(zq_context* c, int count, const int* n2, double* c
   if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -7;
   if (count == 0) return ZQ_OK;
   if ((rc = with_device(c))) return rc;
+  CaptureScope cap(c);
+  if (cap.rc) return cap.rc;
+  if (cap.active && info) return ZQ_ERR_GRAPH;   // info makes the call wait for the result
   return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', info, nullptr);
 }
 
@@ -833,7 +935,7 @@ int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, dou
   const bool vec = dZ != nullptr;
   if ((rc = make_plan(c, n, 32, vec, &p))) return rc;
   hipStream_t s = c->stream;
-  if (vec) ZQ_HIP_CHECK(hipMemsetAsync(p.sw.info, 0, sizeof(int), s));
+  if (vec && (rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
   const int zero = 0;
   p.sw.stage = &c->stage;
   if ((rc = zq::stage_begin(c->stage))) return rc;

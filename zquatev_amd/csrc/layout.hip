@@ -6,6 +6,7 @@
 // Out: X_ij = (U_ij, V_ij) is written back as [[U, -conj V], [V, conj U]]
 //      (zquatev.cc:86-98), columns i and i + n forming the Kramers pair.
 #include "zq_internal.h"
+#include <algorithm>
 
 namespace zq {
 
@@ -116,6 +117,23 @@ __global__ void __launch_bounds__(256) k_gen_lower(int n, unsigned long long see
 int gen_lower_uniform(int n, unsigned long long seed, quat* M, int ld, hipStream_t s) {
   if (n <= 0) return ZQI_OK;
   hipLaunchKernelGGL(k_gen_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, seed, M, ld);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero8(unsigned long long* __restrict__ p, size_t nw) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nw; i += (size_t)gridDim.x * 256) p[i] = 0ull;
+}
+__global__ void __launch_bounds__(256) k_fill_zero4(unsigned* __restrict__ p, size_t nw) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nw; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
+
+int fill_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return ZQI_OK;
+  const bool w8 = (bytes % 8 == 0) && (reinterpret_cast<uintptr_t>(p) % 8 == 0);
+  const size_t nw = w8 ? bytes / 8 : bytes / 4;
+  const unsigned grid = (unsigned)std::min<size_t>((nw + 255) / 256, 8192);
+  if (w8) hipLaunchKernelGGL(k_fill_zero8, dim3(grid), dim3(256), 0, s, static_cast<unsigned long long*>(p), nw);
+  else hipLaunchKernelGGL(k_fill_zero4, dim3(grid), dim3(256), 0, s, static_cast<unsigned*>(p), nw);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

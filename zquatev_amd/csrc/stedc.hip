@@ -792,7 +792,7 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
   }
   const int nleaf = (int)leaves.size() / 4;
   const int ldz = ntot;
-  ZQ_HIP_CHECK(hipMemsetAsync(sw.Z, 0, (size_t)ntot * lq * sizeof(double), s));
+  if ((rc = fill_zero(sw.Z, (size_t)ntot * lq * sizeof(double), s))) return rc;
   if (!splits.empty())
     hipLaunchKernelGGL(k_dc_tear, dim3(cdiv((int)splits.size(), 256)), dim3(256), 0, s, (int)splits.size(), isplit,
                        sw.es, sw.lam);

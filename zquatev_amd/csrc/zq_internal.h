@@ -225,6 +225,7 @@ __device__ __forceinline__ quat wave_sum(quat v) {
 #define ZQI_ERR_HIP (-1001)
 #define ZQI_ERR_NOMEM (-1002)
 #define ZQI_ERR_ARG (-1003)
+#define ZQI_ERR_GRAPH (-1004)
 
 #define ZQ_HIP_CHECK(expr)                                                                  \
   do {                                                                                      \
@@ -343,6 +344,12 @@ struct Staging {
   hipEvent_t done[2] = {nullptr, nullptr};
   int slot = 0;
   size_t used = 0;
+  // While the stream is being captured into a graph, uploads are recorded as
+  // copies from this pinned pool instead (the graph re-reads them at every
+  // replay, so they must stay as written); nothing is allocated or waited on.
+  bool capture = false;
+  char* keep = nullptr;
+  size_t keep_cap = 0, keep_used = 0;
 };
 int stage_begin(Staging& st);
 int stage_upload(Staging& st, const void* src, size_t bytes, void* dst, hipStream_t s);
@@ -404,6 +411,8 @@ int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* in
 int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s);
 int store_vectors(int n, const quat* X, int ldx, double* D, int ld2, hipStream_t s);
 int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
+// zero `bytes` (a multiple of 4) at p with a kernel (no memset node in graphs)
+int fill_zero(void* p, size_t bytes, hipStream_t s);
 // synthetic U(-1,1) lower triangle (real diagonal) for timing runs
 int gen_lower_uniform(int n, unsigned long long seed, quat* M, int ld, hipStream_t s);
 
