@@ -173,13 +173,10 @@ struct CaptureScope {
 };
 
 // reflectors per back-transform block (a multiple of the panel width)
-// Reflectors per compact-WY block of the back-transform: one width for every
-// n, so that a batch member (laid out for its group's largest n) applies the
-// same blocks as its solo solve; 128 measured as fast as 64 for small n.
-inline int backtr_width(int n) {
-  (void)n;
-  return 128;
-}
+// Reflectors per compact-WY block of the back-transform.  Batches group
+// their problems by this width (as by panel width), so that a member applies
+// the same blocks as its solo solve.
+inline int backtr_width(int n) { return n > 1024 ? 128 : 64; }
 
 struct Plan {
   quat* M;
@@ -501,7 +498,8 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
 // call.
 struct Groups {
   std::vector<std::vector<zq::BatchProb>> probs;   // per group, sizes descending
-  std::vector<int> nb;
+  std::vector<int> nb;                             // panel width of each group
+  std::vector<int> wb;                             // back-transform block width of each group
   size_t need = 0;                                 // workspace bytes of the largest group
 };
 
@@ -512,10 +510,12 @@ Groups make_groups(const zq_context* ctx, int count, const int* n2, double* cons
     if (n2[i] == 0) continue;
     const int n = n2[i] / 2;
     const int nb = panel_width(ctx, n);
+    const int wb = backtr_width(n);
     size_t g = 0;
-    while (g < G.nb.size() && G.nb[g] != nb) ++g;
+    while (g < G.nb.size() && !(G.nb[g] == nb && G.wb[g] == wb)) ++g;
     if (g == G.nb.size()) {
       G.nb.push_back(nb);
+      G.wb.push_back(wb);
       G.probs.emplace_back();
     }
     G.probs[g].push_back(zq::BatchProb{D ? D[i] : nullptr, eig ? eig[i] : nullptr, ld2 ? ld2[i] : n2[i], n, 0, i});

@@ -118,6 +118,13 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, int wbt, const quat* __r
   __shared__ quat gs[TB][128];
   const int r = threadIdx.x;
   for (int J = 0; J < wbt; J += TB) {
+    if (J >= wdt) {   // columns past the block's reflectors: identity (uniform branch)
+      if (r < wbt)
+#pragma unroll
+        for (int c = 0; c < TB; ++c)
+          qstore(T + r + (size_t)(J + c) * wbt, r == J + c ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
+      continue;
+    }
     for (int idx = threadIdx.x; idx < TB * wbt; idx += 256) {
       const int c = idx / wbt, l = idx - c * wbt;
       const int jj = J + c;

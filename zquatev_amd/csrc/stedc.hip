@@ -32,7 +32,7 @@
 
 namespace zq {
 
-constexpr int LEAF = 32;
+constexpr int LEAF = 16;
 constexpr double DEPS = DBL_EPSILON * 0.5;   // unit roundoff
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -80,60 +80,72 @@ __global__ void k_dc_tear(int nsplit, const int* __restrict__ splits, const doub
 // leaf's eigenvector block and applies each rotation to it.  A leaf is
 // (lo, hi, problem base, problem index).
 // --------------------------------------------------------------------------
+// Element i (wave-uniform) of a value held one per lane.
+__device__ __forceinline__ double lane_get(double v, int i) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+  return __hiloint2double(hi, lo);
+}
+
+// One leaf per wavefront.  The tridiagonal lives in registers, d[k] and e[k]
+// in lane k; every lane runs the (wave-uniform) QL recurrence redundantly,
+// reading element i with readlane, and the owning lane keeps an update, so
+// the sequential chain has no barrier and no LDS access.  Lane k owns row k
+// of the leaf's eigenvectors (in LDS), to which it applies each rotation.
 __global__ void __launch_bounds__(64) k_dc_leaf(const int* __restrict__ leaves, double* __restrict__ lam,
                                                 const double* __restrict__ es, double* __restrict__ Z, int ldz,
                                                 int* __restrict__ info) {
-  __shared__ double dd[LEAF], ee[LEAF];
   __shared__ double zz[LEAF][LEAF + 1];
   const int lo = leaves[4 * blockIdx.x], hi = leaves[4 * blockIdx.x + 1];
   const int ob = leaves[4 * blockIdx.x + 2], pr = leaves[4 * blockIdx.x + 3];
   const int m = hi - lo;
   const int k = threadIdx.x;
+  double dk = 0.0, ek = 0.0;
   if (k < m) {
-    dd[k] = lam[lo + k];
-    ee[k] = (k < m - 1) ? es[lo + k] : 0.0;
+    dk = lam[lo + k];
+    ek = (k < m - 1) ? es[lo + k] : 0.0;
     for (int i = 0; i < m; ++i) zz[k][i] = (i == k) ? 1.0 : 0.0;
   }
-  __syncthreads();
   int fail = 0;
   for (int l = 0; l < m; ++l) {
     int iter = 0;
     int mm;
     while (true) {
       for (mm = l; mm < m - 1; ++mm) {
-        const double s = fabs(dd[mm]) + fabs(dd[mm + 1]);
-        if (fabs(ee[mm]) <= DEPS * s) break;
+        const double s = fabs(lane_get(dk, mm)) + fabs(lane_get(dk, mm + 1));
+        if (fabs(lane_get(ek, mm)) <= DEPS * s) break;
       }
       if (mm == l) break;
       if (++iter > 60) { fail = 1; break; }
-      double g = (dd[l + 1] - dd[l]) / (2.0 * ee[l]);
+      const double el = lane_get(ek, l);
+      double g = (lane_get(dk, l + 1) - lane_get(dk, l)) / (2.0 * el);
       double r = hypot(g, 1.0);
-      g = dd[mm] - dd[l] + ee[l] / (g + copysign(r, g));
+      g = lane_get(dk, mm) - lane_get(dk, l) + el / (g + copysign(r, g));
       double s = 1.0, c = 1.0, p = 0.0;
       int i;
       bool early = false;
       for (i = mm - 1; i >= l; --i) {
-        const double f = s * ee[i];
-        const double b = c * ee[i];
-        r = hypot(f, g);
-        __syncthreads();
-        if (k == 0) ee[i + 1] = r;
-        __syncthreads();
+        const double ei = lane_get(ek, i);
+        const double f = s * ei;
+        const double b = c * ei;
+        // |(f, g)|: the plain square root unless the squares could over- or
+        // underflow (the chain of rotations is the leaf's critical path)
+        const double rr = f * f + g * g;
+        r = (rr > 1e-280 && rr < 1e280) ? sqrt(rr) : hypot(f, g);
+        if (k == i + 1) ek = r;
         if (r == 0.0) {
-          __syncthreads();
-          if (k == 0) { dd[i + 1] -= p; ee[mm] = 0.0; }
-          __syncthreads();
+          if (k == i + 1) dk -= p;
+          if (k == mm) ek = 0.0;
           early = true;
           break;
         }
-        s = f / r;
-        c = g / r;
-        g = dd[i + 1] - p;
-        r = (dd[i] - g) * s + 2.0 * c * b;
+        const double ir = 1.0 / r;
+        s = f * ir;
+        c = g * ir;
+        g = lane_get(dk, i + 1) - p;
+        r = (lane_get(dk, i) - g) * s + 2.0 * c * b;
         p = s * r;
-        __syncthreads();
-        if (k == 0) dd[i + 1] = g + p;
-        __syncthreads();
+        if (k == i + 1) dk = g + p;
         g = c * r - b;
         if (k < m) {
           const double zf = zz[k][i + 1];
@@ -143,32 +155,32 @@ __global__ void __launch_bounds__(64) k_dc_leaf(const int* __restrict__ leaves, 
         }
       }
       if (early) continue;
-      __syncthreads();
-      if (k == 0) { dd[l] -= p; ee[l] = g; ee[mm] = 0.0; }
-      __syncthreads();
+      if (k == l) { dk -= p; ek = g; }
+      if (k == mm) ek = 0.0;
     }
     if (fail) break;
   }
   // selection sort ascending, columns follow
   for (int i = 0; i < m - 1; ++i) {
     int kmin = i;
-    double vmin = dd[i];
-    for (int j = i + 1; j < m; ++j)
-      if (dd[j] < vmin) { vmin = dd[j]; kmin = j; }
+    double vmin = lane_get(dk, i);
+    for (int j = i + 1; j < m; ++j) {
+      const double dj = lane_get(dk, j);
+      if (dj < vmin) { vmin = dj; kmin = j; }
+    }
     if (kmin != i) {
-      __syncthreads();
-      if (k == 0) { dd[kmin] = dd[i]; dd[i] = vmin; }
+      const double di = lane_get(dk, i);
+      if (k == kmin) dk = di;
+      if (k == i) dk = vmin;
       if (k < m) {
         const double t = zz[k][i];
         zz[k][i] = zz[k][kmin];
         zz[k][kmin] = t;
       }
-      __syncthreads();
     }
   }
-  __syncthreads();
   if (k < m) {
-    lam[lo + k] = dd[k];
+    lam[lo + k] = dk;
     for (int i = 0; i < m; ++i) Z[(lo + k) + (size_t)(lo - ob + i) * ldz] = zz[k][i];
   }
   if (fail && k == 0) atomicMax(info + pr, lo - ob + 1);
