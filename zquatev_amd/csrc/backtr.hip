@@ -97,47 +97,59 @@ __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__
 // T = (I + triu(G, 1))^{-1} for each block: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
 // T[j, j] = 1, zero below (the compact-WY recurrence with every tau = 1).
 // One workgroup per block (blockIdx.x), problem blockIdx.z.
-__global__ void __launch_bounds__(256) k_tfactor(int n, int wbt, const quat* __restrict__ Gall,
-                                                 quat* __restrict__ Tall, ZB zb) {
+template <int WB>
+__global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__ Gall, quat* __restrict__ Tall,
+                                                 ZB zb) {
   const int blk = blockIdx.x;
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     Gall = zoff(Gall, zb); Tall = zoff(Tall, zb);
   }
-  if (blk >= (n - 1 + wbt - 1) / wbt) return;
-  const int p = blk * wbt;
-  const int wdt = min(wbt, n - 1 - p);
-  const quat* G = Gall + (size_t)blk * wbt * wbt;
-  quat* T = Tall + (size_t)blk * wbt * wbt;
+  if (blk >= (n - 1 + WB - 1) / WB) return;
+  const int p = blk * WB;
+  const int wdt = min(WB, n - 1 - p);
+  const quat* G = Gall + (size_t)blk * WB * WB;
+  quat* T = Tall + (size_t)blk * WB * WB;
   // Columns in groups of TB: thread r (its row of T) first applies the
   // columns l < J of the rows it owns (one read of T(r, l) serves the group's
   // TB columns, whose G entries are staged in LDS), then runs the recurrence
   // inside the group from registers.  Every entry sums l = r .. jj-1 in
-  // increasing order, as the column-by-column recurrence does.
+  // increasing order, as the column-by-column recurrence does.  Blocks of 64
+  // keep T itself in LDS (its rows are re-read for every later group) and
+  // write it out at the end; blocks of 128 do not fit and use global memory.
   constexpr int TB = 8;
-  __shared__ quat gs[TB][128];
+  constexpr bool INL = WB <= 64;
+  __shared__ quat gs[TB][WB];
+  __shared__ quat Ts[INL ? WB : 1][INL ? WB : 1];   // Ts[col][row]
+  auto tget = [&](int row, int col) -> quat {
+    if constexpr (INL) return Ts[col][row];
+    else return qload(T + row + (size_t)col * WB);
+  };
+  auto tset = [&](int row, int col, const quat& v) {
+    if constexpr (INL) Ts[col][row] = v;
+    else qstore(T + row + (size_t)col * WB, v);
+  };
   const int r = threadIdx.x;
-  for (int J = 0; J < wbt; J += TB) {
+  for (int J = 0; J < WB; J += TB) {
     if (J >= wdt) {   // columns past the block's reflectors: identity (uniform branch)
-      if (r < wbt)
+      if (r < WB)
 #pragma unroll
-        for (int c = 0; c < TB; ++c)
-          qstore(T + r + (size_t)(J + c) * wbt, r == J + c ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
+        for (int c = 0; c < TB; ++c) tset(r, J + c, r == J + c ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
       continue;
     }
-    for (int idx = threadIdx.x; idx < TB * wbt; idx += 256) {
-      const int c = idx / wbt, l = idx - c * wbt;
+    for (int idx = threadIdx.x; idx < TB * WB; idx += 256) {
+      const int c = idx / WB, l = idx - c * WB;
       const int jj = J + c;
-      gs[c][l] = (jj < wdt && l < jj) ? qload(G + l + (size_t)jj * wbt) : qzero();
+      gs[c][l] = (jj < wdt && l < jj) ? qload(G + l + (size_t)jj * WB) : qzero();
     }
     __syncthreads();
-    if (r < wbt) {
+    if (r < WB) {
       quat acc[TB];
 #pragma unroll
       for (int c = 0; c < TB; ++c) acc[c] = qzero();
 #pragma unroll 2
       for (int l = r; l < J; ++l) {
-        const quat t = qload(T + r + (size_t)l * wbt);
+        const quat t = tget(r, l);
 #pragma unroll
         for (int c = 0; c < TB; ++c) qfma(acc[c], t, gs[c][l]);
       }
@@ -151,10 +163,17 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, int wbt, const quat* __r
           if (r <= J + cc) qfma(a, tb[cc], gs[c][J + cc]);
         if (jj < wdt && r < jj) tb[c] = qscale(a, -1.0);
         else tb[c] = (r == jj) ? quat{1.0, 0.0, 0.0, 0.0} : qzero();
-        if (jj < wbt) qstore(T + r + (size_t)jj * wbt, tb[c]);
+        tset(r, jj, tb[c]);
       }
     }
     __syncthreads();
+  }
+  if constexpr (INL) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < WB * WB; idx += 256) {
+      const int col = idx / WB, row = idx - col * WB;
+      qstore(T + row + (size_t)col * WB, Ts[col][row]);
+    }
   }
 }
 
@@ -297,7 +316,9 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
   // phase A launches
   if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, false, s))) return rc;
   hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(wbt * wbt, 256), 1, ngs), dim3(256), 0, s, sums);
-  hipLaunchKernelGGL(k_tfactor, dim3(nblk, 1, nprob), dim3(256), 0, s, L, wbt, w.G, w.T, zb);
+  if (wbt == 64) hipLaunchKernelGGL(k_tfactor<64>, dim3(nblk, 1, nprob), dim3(256), 0, s, L, w.G, w.T, zb);
+  else if (wbt == 128) hipLaunchKernelGGL(k_tfactor<128>, dim3(nblk, 1, nprob), dim3(256), 0, s, L, w.G, w.T, zb);
+  else return ZQI_ERR_ARG;
   if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, false, s))) return rc;
   // phase B launches
   for (const Step& q : steps) {
