@@ -417,6 +417,7 @@ struct BatchPlan {
   double* wall = nullptr;
   zq::StedcWork sw;
   zq::QGemmTask* rtasks = nullptr;
+  int* tstart = nullptr;
   zq::QGemmTask* btasks = nullptr;
   zq::SumTask* sums = nullptr;
 };
@@ -442,6 +443,7 @@ size_t batch_layout(int nprob, const int* nh, int nb, bool vectors, char* base, 
   bp->wall = b.take<double>(ntot + 1);
   stedc_layout(b, ntot, nprob, L, vectors, &bp->sw);
   bp->rtasks = b.take<zq::QGemmTask>(zq::reduce_batched_tasks(nprob, nh, nb));
+  bp->tstart = b.take<int>(nprob);
   if (vectors) {
     bp->btasks = b.take<zq::QGemmTask>(zq::backtransform_batched_tasks(nprob, nh, backtr_width(L), kNSplit));
     bp->sums = b.take<zq::SumTask>(zq::backtransform_batched_sums(nprob, nh, backtr_width(L)));
@@ -473,7 +475,7 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
   const zq::ZB zb{bp.nvec, bp.zs};
   if ((rc = zq::load_left_batched(nprob, L, bp.probs, bp.slab.M, L, zb, s))) return rc;
   if ((rc = zq::reduce_batched(nprob, nh.data(), zb, bp.slab.M, L, nb, bp.slab.d, bp.slab.e, bp.slab.rw, bp.rtasks,
-                               st, s)))
+                               bp.tstart, st, s)))
     return rc;
   if ((rc = zq::gather_tridiag_batched(nprob, L, bp.probs, bp.slab.d, bp.slab.e, zb, bp.dall, bp.eall, s))) return rc;
   bp.sw.stage = &st;

@@ -31,6 +31,7 @@ constexpr int SW = 32;       // matvec strip width (columns)
 constexpr int SG = 64;       // matvec rows per block (one per lane)
 constexpr int SYT = 512;     // matvec threads: 8 waves x 4 columns
 constexpr int NBMAX = 128;   // largest supported panel width
+constexpr int TAILN = 64;    // trailing size from which one workgroup finishes in LDS (k_tail)
 
 // ceil(a / b) for b > 0, and 0 for a <= 0
 __device__ __forceinline__ int cdiv_d(int a, int b) { return a > 0 ? (a + b - 1) / b : 0; }
@@ -513,6 +514,97 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-resident tail.  Once the trailing matrix has at most TAILN rows, one
+// workgroup per problem finishes the reduction with the whole Hermitian
+// trailing block in LDS (TAILN x TAILN quaternions, 128 KB, both triangles so
+// that every access is a row-contiguous, conflict-free read): per column the
+// reflector, y = A u, K = u^* y / 2, w = y - K u and the rank-2 update
+// A -= u w^* + w u^* at once (unblocked: no panel, no W), separated by
+// barriers instead of kernel launches.  Lane = row, the 4 waves split the
+// columns.  It starts at a panel boundary p, after the trailing update of the
+// panel before it, and writes d and e of columns p .. n-2, d[n-1], and the
+// reflectors u into M below the subdiagonal (the blocked columns'
+// conventions).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_tail(quat* __restrict__ M, int ld, int n, int p, const int* __restrict__ pstart,
+                                              double* __restrict__ d, quat* __restrict__ e, ZB zb) {
+  if (zb.nvec) {
+    n = zb.nvec[blockIdx.z];
+    p = pstart[blockIdx.z];
+    M = zoff(M, zb); d = zoff(d, zb); e = zoff(e, zb);
+  }
+  const int ms = n - p;
+  if (ms < 1 || ms > TAILN) return;   // uniform: a problem without a tail
+  __shared__ quat A[TAILN * TAILN];     // A[j * TAILN + i] = element (i, j)
+  __shared__ quat us[TAILN], ys[TAILN];
+  __shared__ quat part[RB / 64][TAILN];
+  __shared__ double sh[RB / 64];
+  const int tid = threadIdx.x;
+  const int tx = tid & 63, ty = tid >> 6;   // row, column group
+  for (int j = ty; j < ms; j += RB / 64) {
+    if (tx < ms) {
+      quat a = tx >= j ? qload(M + (p + tx) + (size_t)(p + j) * ld) : qconj(qload(M + (p + j) + (size_t)(p + tx) * ld));
+      if (tx == j) a = quat{a.ar, 0.0, 0.0, 0.0};
+      A[j * TAILN + tx] = a;
+    }
+  }
+  __syncthreads();
+  for (int cl = 0; cl + 1 < ms; ++cl) {
+    const int r0 = cl + 1;                  // first row of the trailing block
+    const bool row = tx >= r0 && tx < ms;   // this lane's row is in the trailing block
+    const double v = (ty == 0 && tx > r0 && tx < ms) ? qnorm2(A[cl * TAILN + tx]) : 0.0;
+    const double tail = block_sum(v, sh);
+    quat alpha;
+    const Refl R = reflector(A[cl * TAILN + r0], tail, &alpha);
+    if (tid == 0) {
+      d[p + cl] = A[cl * TAILN + cl].ar;
+      qstore(e + p + cl, alpha);
+    }
+    if (ty == 0 && row) {
+      const quat ui = tx == r0 ? R.u1 : qscale(A[cl * TAILN + tx], R.g);
+      us[tx] = ui;
+      qstore(M + (p + tx) + (size_t)(p + cl) * ld, ui);
+    }
+    __syncthreads();
+    quat acc = qzero();   // this lane's row of A22 u over this wave's columns
+    if (row)
+      for (int jj = r0 + ty; jj < ms; jj += RB / 64) qfma(acc, A[jj * TAILN + tx], us[jj]);
+    part[ty][tx] = acc;
+    __syncthreads();
+    double kv = 0.0;
+    if (ty == 0 && row) {
+      quat y = part[0][tx];
+#pragma unroll
+      for (int k = 1; k < RB / 64; ++k) y = qadd(y, part[k][tx]);
+      ys[tx] = y;
+      const quat u = us[tx];
+      kv = u.ar * y.ar + u.ai * y.ai + u.br * y.br + u.bi * y.bi;   // Re(conj(u_i) y_i)
+    }
+    const double K = 0.5 * block_sum(kv, sh);   // its barriers also publish ys
+    if (ty == 0 && row) ys[tx] = qsub(ys[tx], qscale(us[tx], K));   // w
+    __syncthreads();
+    if (row) {
+      const quat ui = us[tx], wi = ys[tx];
+      for (int jj = r0 + ty; jj < ms; jj += RB / 64) {
+        quat a = A[jj * TAILN + tx];
+        a = qsub(a, qmul(ui, qconj(ys[jj])));
+        a = qsub(a, qmul(wi, qconj(us[jj])));
+        if (jj == tx) a = quat{a.ar, 0.0, 0.0, 0.0};   // Hermitian diagonal: real
+        A[jj * TAILN + tx] = a;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) d[n - 1] = A[(ms - 1) * TAILN + (ms - 1)].ar;
+}
+
+int tail_start(int n, int nb) {
+  if (n <= TAILN) return 0;
+  const int p = (n - TAILN + nb - 1) / nb * nb;
+  return p <= n - 1 ? p : n;
+}
+
 __global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* __restrict__ d, ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
@@ -561,7 +653,8 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
   }
   const int ldw = n;
   const int kpt = kpt_for(nb);
-  for (int p = 0; p < n - 1; p += nb) {
+  const int ts = tail_start(n, nb);   // n: no tail
+  for (int p = 0; p < n - 1 && p < ts; p += nb) {
     const int wdt = (nb < n - 1 - p) ? nb : (n - 1 - p);
     for (int j = 0; j < wdt; ++j) {
       const int c = p + j;
@@ -592,7 +685,8 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
       if (rc) return rc;
     }
   }
-  hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
+  if (ts < n) hipLaunchKernelGGL(k_tail, dim3(1), dim3(RB), 0, s, M, ld, n, ts, nullptr, d, e, ZB{nullptr, 0});
+  else hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -602,13 +696,20 @@ size_t reduce_batched_tasks(int nprob, const int* nh, int nb) {
 }
 
 int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, double* d, quat* e, const ReduceWork& w,
-                   QGemmTask* tasks, Staging& st, hipStream_t s) {
+                   QGemmTask* tasks, int* tstart, Staging& st, hipStream_t s) {
   if (nprob <= 0) return ZQI_OK;
   if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
   const int L = nh[0];
   for (int z = 1; z < nprob; ++z)
     if (nh[z] > nh[z - 1] || nh[z] < 1) return ZQI_ERR_ARG;
   const int ldw = ld;
+  // problem z runs the blocked columns c < lim[z] in lockstep and its tail
+  // (k_tail) from ts[z]; both are non-increasing in z (sizes descending)
+  std::vector<int> ts(nprob), lim(nprob);
+  for (int z = 0; z < nprob; ++z) {
+    ts[z] = tail_start(nh[z], nb);
+    lim[z] = std::min(ts[z], nh[z] - 1);
+  }
   const int npan = L > 1 ? cdiv(L - 1, nb) : 0;
   // trailing-update tasks of every panel, problem by problem: the task of
   // problem z in panel P is what reduce() launches for z after that panel
@@ -618,6 +719,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     const int p = P * nb;
     toff[P] = (int)tt.size();
     for (int z = 0; z < nprob && nh[z] - 1 > p; ++z) {
+      if (p >= ts[z]) continue;
       const int n = nh[z];
       const int wdt = std::min(nb, n - 1 - p);
       const int r0 = p + wdt, mt = n - r0;
@@ -634,10 +736,12 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
   if (tt.size() > reduce_batched_tasks(nprob, nh, nb)) return ZQI_ERR_ARG;
   int rc = stage_upload(st, tt.data(), tt.size() * sizeof(QGemmTask), tasks, s);
   if (rc) return rc;
+  if ((rc = stage_upload(st, ts.data(), nprob * sizeof(int), tstart, s))) return rc;
   const int kpt = kpt_for(nb);
-  int act = nprob;   // problems with a column c: n - 1 > c (a prefix, sizes descending)
-  for (int c = 0; c < L - 1; ++c) {
-    while (act > 0 && nh[act - 1] - 1 <= c) --act;
+  const int cend = lim[0];
+  int act = nprob;   // problems with a blocked column c: c < lim[z] (a prefix)
+  for (int c = 0; c < cend; ++c) {
+    while (act > 0 && lim[act - 1] <= c) --act;
     const int p = (c / nb) * nb, j = c - p;
     const int nrow = L - c;
     const int nblk = cdiv(nrow, RQ);
@@ -650,13 +754,15 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
                        w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, w.QF, zb);
     launch_wcol(kpt, sz, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
                 w.scal, w.QF, w.part, w.TSP, w.ldtsp, zb);
-    if (j == nb - 1 || c == L - 2) {
+    if (j == nb - 1 || c == cend - 1) {
       const int P = c / nb;
       const int cnt = toff[P + 1] - toff[P];
-      if ((rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, true, s))) return rc;
+      if (cnt > 0 && (rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, true, s))) return rc;
     }
   }
+  // problems without a tail take d[n-1] here; the tails overwrite theirs
   hipLaunchKernelGGL(k_last_diag, dim3(1, 1, nprob), dim3(64), 0, s, M, ld, L, d, zb);
+  hipLaunchKernelGGL(k_tail, dim3(1, 1, nprob), dim3(RB), 0, s, M, ld, L, 0, tstart, d, e, zb);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

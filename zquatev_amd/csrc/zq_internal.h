@@ -299,10 +299,11 @@ struct QGemmTask;
 // host copy of the sizes, zb.nvec the device copy); every buffer of problem z
 // lies zb.zs bytes after problem z-1's (the pointers given are problem 0's,
 // ld = nh[0]).  tasks: device room for the trailing-update task table
-// (reduce_batched_tasks entries).  Each problem's result equals its solo reduce().
+// (reduce_batched_tasks entries); tstart: device room for nprob ints.  Each
+// problem's result equals its solo reduce().
 size_t reduce_batched_tasks(int nprob, const int* nh, int nb);
 int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, double* d, quat* e, const ReduceWork& w,
-                   QGemmTask* tasks, Staging& st, hipStream_t s);
+                   QGemmTask* tasks, int* tstart, Staging& st, hipStream_t s);
 
 // stedc.hip.  Several independent tridiagonals (a forest) are solved
 // together: problem z owns the global index range [o_z, o_z + n_z) of every
