@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-values-only", action="store_true", help="skip the eigenvalues-only (jobz='N') summary")
+    ap.add_argument("--no-host-entry", action="store_true",
+                    help="skip the host-memory (drop-in, PCIe-inclusive) summary")
     ap.add_argument("--workload", choices=("single", "batched", "serving"), default="single",
                     help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only; "
                          "serving: the Poisson request stream through the server and the plain loop")
@@ -515,6 +517,23 @@ def main():
                 "phases_ms": {k: round(v, 3) for k, v in ctx.last_timings().items()},
                 "max_abs_diff_vs_vectors": float((eig_n - eig_v).abs().max().item())}
 
+    # the drop-in host-memory entry point (ts::zquatev semantics) on the same
+    # input: host -> device copy, solve, device -> host copy (PCIe-inclusive)
+    host = None
+    if not args.no_host_entry:
+        th = []
+        for r in range(2):
+            Dh = np.asfortranarray(D.copy())
+            t1 = time.perf_counter()
+            info, eig_h = ctx.zquatev(Dh)
+            th.append(time.perf_counter() - t1)
+            if info != 0:
+                raise RuntimeError(f"zq_zquatev_ctx failed ({info})")
+        host = {"ms": round(1000.0 * th[-1], 3), "gflops_pcie_inclusive": round(flops / th[-1] / 1e9, 1),
+                "bytes_h2d": int(D.nbytes) // 2, "bytes_d2h": int(D.nbytes),
+                "max_abs_diff_vs_device": float(np.abs(eig_h - eig.cpu().numpy()).max())}
+        del Dh
+
     # instrumented pass: per-kernel-class HIP event times on the library's stream
     ctx.set_instrument(True)
     step()
@@ -584,6 +603,7 @@ def main():
         "check": check,
         "batched": batched,
         "values_only": vals,
+        "host_entry": host,
     }
     print(json.dumps(line))
     return 0

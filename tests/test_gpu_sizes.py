@@ -89,6 +89,26 @@ def test_zquatev_large(gpu_ctx, n2, nb):
     inv.assert_solution(H, X, eig)
 
 
+@pytest.mark.parametrize("n2,pad", [(4096, 6), (3000, 1)])
+def test_zquatev_host_chunked(gpu_ctx, n2, pad):
+    """Host entry at sizes whose copies span several pinned chunks, with and
+    without rows of padding below the matrix (ld2 > n2): the padding rows and
+    the columns beyond n2 are left untouched."""
+    _say(f"case: zquatev host chunked 2n={n2} pad={pad}")
+    n = n2 // 2
+    D = gen.uniform_case(n, 7000 + n2 + pad)
+    H = inv.full_h(D, n)
+    X = np.zeros((n2 + pad, n2 + 1), dtype=np.complex128, order="F")
+    X[:n2, :n2] = D
+    X[n2:, :n2] = 7.0 - 3.0j
+    X[:, n2] = -1.5 + 2.5j
+    info, eig = gpu_ctx.zquatev(X, n2)
+    assert info == 0
+    assert np.all(X[n2:, :n2] == 7.0 - 3.0j)
+    assert np.all(X[:, n2] == -1.5 + 2.5j)
+    inv.assert_solution(H, np.asfortranarray(X[:n2, :n2]), eig)
+
+
 def _device_checks(Dcols, n, eig, H, ncols=64):
     """Size-independent checks of a device solve: trace identity, residual and
     orthonormality of sampled eigenpairs, Kramers structure of the sampled columns."""

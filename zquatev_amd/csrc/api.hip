@@ -16,6 +16,7 @@
 #include <complex>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 namespace zq {
@@ -38,6 +39,9 @@ struct zq_context {
   int nb_override = 0;
   int tuned_nb[32] = {};      // autotuned panel width per bucket floor(log2 n) (0: none)
   bool ws_locked = false;     // captured graphs reference ws / binfo: no reallocation until released
+  char* pin[2] = {nullptr, nullptr};   // pinned chunks of the host entry points' copies
+  size_t pin_cap = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   void* ws = nullptr;
   size_t ws_bytes = 0;
   void* dD = nullptr;       // staging copy of D for the host-memory entry points
@@ -600,6 +604,87 @@ zq_context* g_ctx = nullptr;
 
 }  // namespace
 
+// ---- host <-> device copies of the host entry points --------------------
+// The caller's D is pageable memory.  Columns go through two pinned chunks of
+// the context: several host threads copy one chunk while the DMA engine moves
+// the other (the HIP runtime's own staging of a pageable copy is one thread).
+constexpr size_t kPinChunk = size_t(32) << 20;
+
+int pin_ready(zq_context* c) {
+  if (c->pin[0]) return ZQ_OK;
+  for (int k = 0; k < 2; ++k) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, kPinChunk, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return ZQ_ERR_NOMEM;
+    }
+    c->pin[k] = static_cast<char*>(h);
+    if (hipEventCreateWithFlags(&c->pin_ev[k], hipEventDisableTiming) != hipSuccess) return ZQ_ERR_HIP;
+  }
+  c->pin_cap = kPinChunk;
+  return ZQ_OK;
+}
+
+// ncols columns of `bytes` each: column j from src + j * sstride to dst + j * dstride
+void copy_cols(char* dst, size_t dstride, const char* src, size_t sstride, size_t bytes, int ncols) {
+  const int nt = std::max(1, std::min(8, ncols));
+  auto work = [&](int t) {
+    const int a = (int)((long)ncols * t / nt), b = (int)((long)ncols * (t + 1) / nt);
+    for (int j = a; j < b; ++j) std::memcpy(dst + (size_t)j * dstride, src + (size_t)j * sstride, bytes);
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+}
+
+// host (ncols columns of colb bytes at stride hstride) -> device (contiguous columns)
+int h2d_cols(zq_context* c, char* dev, const char* host, size_t hstride, size_t colb, int ncols, hipStream_t s) {
+  int rc = pin_ready(c);
+  if (rc) return rc;
+  const int per = std::max(1, (int)(c->pin_cap / colb));
+  if ((size_t)per * colb > c->pin_cap) {   // a column larger than a chunk: the runtime's copy
+    ZQ_HIP_CHECK(hipMemcpy2DAsync(dev, colb, host, hstride, colb, ncols, hipMemcpyHostToDevice, s));
+    return ZQ_OK;
+  }
+  for (int j0 = 0, k = 0; j0 < ncols; j0 += per, k ^= 1) {
+    const int j1 = std::min(ncols, j0 + per);
+    ZQ_HIP_CHECK(hipEventSynchronize(c->pin_ev[k]));   // the chunk's previous DMA has read it
+    copy_cols(c->pin[k], colb, host + (size_t)j0 * hstride, hstride, colb, j1 - j0);
+    ZQ_HIP_CHECK(hipMemcpyAsync(dev + (size_t)j0 * colb, c->pin[k], (size_t)(j1 - j0) * colb, hipMemcpyHostToDevice, s));
+    ZQ_HIP_CHECK(hipEventRecord(c->pin_ev[k], s));
+  }
+  return ZQ_OK;
+}
+
+// device (contiguous columns) -> host (stride hstride); synchronous
+int d2h_cols(zq_context* c, char* host, size_t hstride, const char* dev, size_t colb, int ncols, hipStream_t s) {
+  int rc = pin_ready(c);
+  if (rc) return rc;
+  const int per = std::max(1, (int)(c->pin_cap / colb));
+  if ((size_t)per * colb > c->pin_cap) {
+    ZQ_HIP_CHECK(hipMemcpy2DAsync(host, hstride, dev, colb, colb, ncols, hipMemcpyDeviceToHost, s));
+    ZQ_HIP_CHECK(hipStreamSynchronize(s));
+    return ZQ_OK;
+  }
+  const int nch = (ncols + per - 1) / per;
+  auto issue = [&](int q) -> int {
+    const int j0 = q * per, j1 = std::min(ncols, j0 + per);
+    ZQ_HIP_CHECK(hipMemcpyAsync(c->pin[q & 1], dev + (size_t)j0 * colb, (size_t)(j1 - j0) * colb,
+                                hipMemcpyDeviceToHost, s));
+    ZQ_HIP_CHECK(hipEventRecord(c->pin_ev[q & 1], s));
+    return ZQ_OK;
+  };
+  if ((rc = issue(0))) return rc;
+  for (int q = 0; q < nch; ++q) {
+    if (q + 1 < nch && (rc = issue(q + 1))) return rc;   // the other chunk, while this one drains
+    ZQ_HIP_CHECK(hipEventSynchronize(c->pin_ev[q & 1]));
+    const int j0 = q * per, j1 = std::min(ncols, j0 + per);
+    copy_cols(host + (size_t)j0 * hstride, hstride, c->pin[q & 1], colb, colb, j1 - j0);
+  }
+  return ZQ_OK;
+}
+
 extern "C" {
 
 int zq_version(void) { return 100; }
@@ -688,6 +773,10 @@ int zq_destroy(zq_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   zq::stage_free(c->stage);
+  for (int k = 0; k < 2; ++k) {
+    if (c->pin[k]) (void)hipHostFree(c->pin[k]);
+    if (c->pin_ev[k]) (void)hipEventDestroy(c->pin_ev[k]);
+  }
   if (c->ws) (void)hipFree(c->ws);
   if (c->dD) (void)hipFree(c->dD);
   if (c->deig) (void)hipFree(c->deig);
@@ -829,9 +918,11 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
   if ((rc = ensure(reinterpret_cast<void*&>(c->deig), c->deig_bytes, (size_t)n * sizeof(double)))) return rc;
   double* dD = static_cast<double*>(c->dD);
   hipStream_t s = c->stream;
-  ZQ_HIP_CHECK(hipMemcpy2DAsync(dD, colb, D, (size_t)nld2 * 16, colb, n, hipMemcpyHostToDevice, s));
+  if ((rc = h2d_cols(c, reinterpret_cast<char*>(dD), reinterpret_cast<const char*>(D), (size_t)nld2 * 16, colb, n, s)))
+    return rc;
   if ((rc = solve_device(c, n, dD, n2, c->deig, true))) return rc;
-  ZQ_HIP_CHECK(hipMemcpy2DAsync(D, (size_t)nld2 * 16, dD, colb, colb, n2, hipMemcpyDeviceToHost, s));
+  if ((rc = d2h_cols(c, reinterpret_cast<char*>(D), (size_t)nld2 * 16, reinterpret_cast<const char*>(dD), colb, n2, s)))
+    return rc;
   ZQ_HIP_CHECK(hipMemcpyAsync(eig, c->deig, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
   int info = 0;
   if (c->info_dev) ZQ_HIP_CHECK(hipMemcpyAsync(&info, c->info_dev, sizeof(int), hipMemcpyDeviceToHost, s));
