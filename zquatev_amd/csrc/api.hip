@@ -625,12 +625,68 @@ int pin_ready(zq_context* c) {
   return ZQ_OK;
 }
 
-// ncols columns of `bytes` each: column j from src + j * sstride to dst + j * dstride
-void copy_cols(char* dst, size_t dstride, const char* src, size_t sstride, size_t bytes, int ncols) {
-  const int nt = std::max(1, std::min(8, ncols));
+// A run of ncols columns of colb bytes: column j at host + j * hstride on the
+// host side, at dev + j * colb on the device side (columns contiguous there).
+struct ColRun {
+  char* dev;
+  char* host;
+  size_t hstride, colb;
+  int ncols;
+  size_t poff;   // offset of its first column in the pinned chunk
+};
+
+// Greedy packing of the runs, in order, into groups of at most `cap` bytes;
+// a run is split between groups at a column boundary.  Runs whose columns are
+// larger than a chunk are returned in `big` (copied by the runtime).
+void plan_groups(const std::vector<ColRun>& runs, size_t cap, std::vector<std::vector<ColRun>>& groups,
+                 std::vector<ColRun>& big) {
+  size_t used = cap;
+  for (ColRun r : runs) {
+    if (r.ncols <= 0) continue;
+    if (r.colb > cap) {
+      big.push_back(r);
+      continue;
+    }
+    while (r.ncols > 0) {
+      int fit = (int)std::min<size_t>((size_t)r.ncols, (cap - used) / r.colb);
+      if (fit == 0) {
+        groups.emplace_back();
+        used = 0;
+        continue;
+      }
+      ColRun q = r;
+      q.ncols = fit;
+      q.poff = used;
+      groups.back().push_back(q);
+      used += (size_t)fit * r.colb;
+      r.dev += (size_t)fit * r.colb;
+      r.host += (size_t)fit * r.hstride;
+      r.ncols -= fit;
+    }
+  }
+}
+
+// host columns <-> pinned chunk for one group, split over up to 8 threads
+void pack_group(const std::vector<ColRun>& g, char* pin, bool to_pinned) {
+  size_t cols = 0, bytes = 0;
+  for (const ColRun& r : g) {
+    cols += (size_t)r.ncols;
+    bytes += (size_t)r.ncols * r.colb;
+  }
+  const int nt = (int)std::max<size_t>(1, std::min<size_t>({(size_t)8, cols, bytes >> 22}));
   auto work = [&](int t) {
-    const int a = (int)((long)ncols * t / nt), b = (int)((long)ncols * (t + 1) / nt);
-    for (int j = a; j < b; ++j) std::memcpy(dst + (size_t)j * dstride, src + (size_t)j * sstride, bytes);
+    const size_t a = cols * t / nt, b = cols * (t + 1) / nt;
+    size_t base = 0;
+    for (const ColRun& r : g) {
+      const size_t lo = std::max(a, base), hi = std::min(b, base + (size_t)r.ncols);
+      for (size_t j = lo; j < hi; ++j) {
+        char* h = r.host + (j - base) * r.hstride;
+        char* p = pin + r.poff + (j - base) * r.colb;
+        if (to_pinned) std::memcpy(p, h, r.colb);
+        else std::memcpy(h, p, r.colb);
+      }
+      base += (size_t)r.ncols;
+    }
   };
   std::vector<std::thread> th;
   for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
@@ -638,50 +694,49 @@ void copy_cols(char* dst, size_t dstride, const char* src, size_t sstride, size_
   for (auto& x : th) x.join();
 }
 
-// host (ncols columns of colb bytes at stride hstride) -> device (contiguous columns)
-int h2d_cols(zq_context* c, char* dev, const char* host, size_t hstride, size_t colb, int ncols, hipStream_t s) {
+// host -> device, asynchronous on s (the host buffers may be reused on return)
+int h2d_runs(zq_context* c, const std::vector<ColRun>& runs, hipStream_t s) {
   int rc = pin_ready(c);
   if (rc) return rc;
-  const int per = std::max(1, (int)(c->pin_cap / colb));
-  if ((size_t)per * colb > c->pin_cap) {   // a column larger than a chunk: the runtime's copy
-    ZQ_HIP_CHECK(hipMemcpy2DAsync(dev, colb, host, hstride, colb, ncols, hipMemcpyHostToDevice, s));
-    return ZQ_OK;
-  }
-  for (int j0 = 0, k = 0; j0 < ncols; j0 += per, k ^= 1) {
-    const int j1 = std::min(ncols, j0 + per);
+  std::vector<std::vector<ColRun>> groups;
+  std::vector<ColRun> big;
+  plan_groups(runs, c->pin_cap, groups, big);
+  for (const ColRun& r : big)
+    ZQ_HIP_CHECK(hipMemcpy2DAsync(r.dev, r.colb, r.host, r.hstride, r.colb, r.ncols, hipMemcpyHostToDevice, s));
+  for (size_t q = 0; q < groups.size(); ++q) {
+    const int k = (int)(q & 1);
     ZQ_HIP_CHECK(hipEventSynchronize(c->pin_ev[k]));   // the chunk's previous DMA has read it
-    copy_cols(c->pin[k], colb, host + (size_t)j0 * hstride, hstride, colb, j1 - j0);
-    ZQ_HIP_CHECK(hipMemcpyAsync(dev + (size_t)j0 * colb, c->pin[k], (size_t)(j1 - j0) * colb, hipMemcpyHostToDevice, s));
+    pack_group(groups[q], c->pin[k], true);
+    for (const ColRun& r : groups[q])
+      ZQ_HIP_CHECK(hipMemcpyAsync(r.dev, c->pin[k] + r.poff, (size_t)r.ncols * r.colb, hipMemcpyHostToDevice, s));
     ZQ_HIP_CHECK(hipEventRecord(c->pin_ev[k], s));
   }
   return ZQ_OK;
 }
 
-// device (contiguous columns) -> host (stride hstride); synchronous
-int d2h_cols(zq_context* c, char* host, size_t hstride, const char* dev, size_t colb, int ncols, hipStream_t s) {
+// device -> host, after the work queued on s; returns when the host holds the data
+int d2h_runs(zq_context* c, const std::vector<ColRun>& runs, hipStream_t s) {
   int rc = pin_ready(c);
   if (rc) return rc;
-  const int per = std::max(1, (int)(c->pin_cap / colb));
-  if ((size_t)per * colb > c->pin_cap) {
-    ZQ_HIP_CHECK(hipMemcpy2DAsync(host, hstride, dev, colb, colb, ncols, hipMemcpyDeviceToHost, s));
-    ZQ_HIP_CHECK(hipStreamSynchronize(s));
-    return ZQ_OK;
-  }
-  const int nch = (ncols + per - 1) / per;
-  auto issue = [&](int q) -> int {
-    const int j0 = q * per, j1 = std::min(ncols, j0 + per);
-    ZQ_HIP_CHECK(hipMemcpyAsync(c->pin[q & 1], dev + (size_t)j0 * colb, (size_t)(j1 - j0) * colb,
-                                hipMemcpyDeviceToHost, s));
-    ZQ_HIP_CHECK(hipEventRecord(c->pin_ev[q & 1], s));
+  std::vector<std::vector<ColRun>> groups;
+  std::vector<ColRun> big;
+  plan_groups(runs, c->pin_cap, groups, big);
+  for (const ColRun& r : big)
+    ZQ_HIP_CHECK(hipMemcpy2DAsync(r.host, r.hstride, r.dev, r.colb, r.colb, r.ncols, hipMemcpyDeviceToHost, s));
+  auto issue = [&](size_t q) -> int {
+    const int k = (int)(q & 1);
+    for (const ColRun& r : groups[q])
+      ZQ_HIP_CHECK(hipMemcpyAsync(c->pin[k] + r.poff, r.dev, (size_t)r.ncols * r.colb, hipMemcpyDeviceToHost, s));
+    ZQ_HIP_CHECK(hipEventRecord(c->pin_ev[k], s));
     return ZQ_OK;
   };
-  if ((rc = issue(0))) return rc;
-  for (int q = 0; q < nch; ++q) {
-    if (q + 1 < nch && (rc = issue(q + 1))) return rc;   // the other chunk, while this one drains
+  if (!groups.empty() && (rc = issue(0))) return rc;
+  for (size_t q = 0; q < groups.size(); ++q) {
+    if (q + 1 < groups.size() && (rc = issue(q + 1))) return rc;   // the other chunk, while this one drains
     ZQ_HIP_CHECK(hipEventSynchronize(c->pin_ev[q & 1]));
-    const int j0 = q * per, j1 = std::min(ncols, j0 + per);
-    copy_cols(host + (size_t)j0 * hstride, hstride, c->pin[q & 1], colb, colb, j1 - j0);
+    pack_group(groups[q], c->pin[q & 1], false);
   }
+  ZQ_HIP_CHECK(hipStreamSynchronize(s));
   return ZQ_OK;
 }
 
@@ -918,11 +973,11 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
   if ((rc = ensure(reinterpret_cast<void*&>(c->deig), c->deig_bytes, (size_t)n * sizeof(double)))) return rc;
   double* dD = static_cast<double*>(c->dD);
   hipStream_t s = c->stream;
-  if ((rc = h2d_cols(c, reinterpret_cast<char*>(dD), reinterpret_cast<const char*>(D), (size_t)nld2 * 16, colb, n, s)))
-    return rc;
+  ColRun run{reinterpret_cast<char*>(dD), reinterpret_cast<char*>(D), (size_t)nld2 * 16, colb, n, 0};
+  if ((rc = h2d_runs(c, {run}, s))) return rc;
   if ((rc = solve_device(c, n, dD, n2, c->deig, true))) return rc;
-  if ((rc = d2h_cols(c, reinterpret_cast<char*>(D), (size_t)nld2 * 16, reinterpret_cast<const char*>(dD), colb, n2, s)))
-    return rc;
+  run.ncols = n2;
+  if ((rc = d2h_runs(c, {run}, s))) return rc;
   ZQ_HIP_CHECK(hipMemcpyAsync(eig, c->deig, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
   int info = 0;
   if (c->info_dev) ZQ_HIP_CHECK(hipMemcpyAsync(&info, c->info_dev, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -960,29 +1015,29 @@ int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D
   hipStream_t s = c->stream;
   std::vector<double*> dptr(count), eptr(count);
   std::vector<int> dld(count);
+  std::vector<ColRun> runs;
   for (int i = 0; i < count; ++i) {
     dptr[i] = reinterpret_cast<double*>(static_cast<char*>(c->dD) + off[i]);
     eptr[i] = c->deig + eoff[i];
     dld[i] = n2[i] > 0 ? n2[i] : 1;
-    const int n = n2[i] / 2;
     if (n2[i] > 0)
-      ZQ_HIP_CHECK(hipMemcpy2DAsync(dptr[i], (size_t)n2[i] * 16, D[i], (size_t)ld2[i] * 16, (size_t)n2[i] * 16, n,
-                                    hipMemcpyHostToDevice, s));
+      runs.push_back({reinterpret_cast<char*>(dptr[i]), reinterpret_cast<char*>(D[i]), (size_t)ld2[i] * 16,
+                      (size_t)n2[i] * 16, n2[i] / 2, 0});
   }
+  if ((rc = h2d_runs(c, runs, s))) return rc;
   std::vector<int> inf(count, 0);
-  rc = solve_batched(c, count, n2, dptr.data(), dld.data(), eptr.data(), true, inf.data(), nullptr);
-  if (rc < 0) return rc;
-  for (int i = 0; i < count; ++i) {
-    if (n2[i] == 0) continue;
-    const int n = n2[i] / 2;
-    ZQ_HIP_CHECK(hipMemcpy2DAsync(D[i], (size_t)ld2[i] * 16, dptr[i], (size_t)n2[i] * 16, (size_t)n2[i] * 16, n2[i],
-                                  hipMemcpyDeviceToHost, s));
-    ZQ_HIP_CHECK(hipMemcpyAsync(eig[i], eptr[i], (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
-  }
-  ZQ_HIP_CHECK(hipStreamSynchronize(s));
+  const int solved = solve_batched(c, count, n2, dptr.data(), dld.data(), eptr.data(), true, inf.data(), nullptr);
+  if (solved < 0) return solved;
+  for (ColRun& r : runs) r.ncols *= 2;
+  // eigenvalues: one run per problem of one "column" of n doubles
+  for (int i = 0; i < count; ++i)
+    if (n2[i] > 0)
+      runs.push_back({reinterpret_cast<char*>(eptr[i]), reinterpret_cast<char*>(eig[i]), 0,
+                      (size_t)(n2[i] / 2) * sizeof(double), 1, 0});
+  if ((rc = d2h_runs(c, runs, s))) return rc;
   if (info)
     for (int i = 0; i < count; ++i) info[i] = inf[i];
-  return rc;
+  return solved;
 }
 
 int zq_zquatev(int n2, double* D, int nld2, double* eig) {
