@@ -226,12 +226,12 @@ void slab_layout(Bump& b, int n, int nb, bool vectors, Plan* p) {
   p->e = b.take<quat>(nmax);
   p->ea = b.take<double>(nmax);
   p->rw.W = b.take<quat>((size_t)n * nb);
-  p->rw.Prow = b.take<quat>((size_t)cdiv(nmax, 32) * n);
-  p->rw.Pcol = b.take<quat>((size_t)cdiv(nmax, 64) * n);
+  p->rw.Prow = b.take<quat>((size_t)zq::matvec_row_slots(nmax) * n);
+  p->rw.Pcol = b.take<quat>((size_t)zq::matvec_col_slots(nmax) * n);
   p->rw.TS = b.take<quat>((size_t)2 * nb);
   p->rw.ldtsp = 8 * nb;
   p->rw.TSP = b.take<double>((size_t)(cdiv(nmax, 16) + 1) * 8 * nb);
-  p->rw.QF = b.take<double>((size_t)(cdiv(nmax, 64) + 1) * (cdiv(nmax, 64) + 2));
+  p->rw.QF = b.take<double>((size_t)zq::matvec_tile_slots(nmax));
   p->rw.part = b.take<double>(cdiv(nmax, 16) + 16);
   p->rw.scal = b.take<double>(16);
   if (vectors) {

@@ -27,9 +27,16 @@ namespace zq {
 
 constexpr int RB = 256;      // threads per block of the row kernels
 constexpr int RQ = 16;       // rows per block of the row kernels (16 threads per row split the K sums)
-constexpr int SW = 32;       // matvec strip width (columns)
 constexpr int SG = 64;       // matvec rows per block (one per lane)
-constexpr int SYT = 512;     // matvec threads: 8 waves x 4 columns
+// Matvec tile shapes: SG rows x SW columns.  Trailing sizes up to TSM use
+// narrow strips (SW = 16, 256 threads: more, smaller blocks, which keeps the
+// small matvecs of batches and of the end of a solve spread over the CUs);
+// larger ones wide strips (SW = 32, 512 threads: fewer row partials for the
+// column kernel to gather).  spg = SG / SW strips per row group.
+constexpr int TSM = 512;
+__host__ __device__ constexpr int matvec_spg(int m) { return m <= TSM ? 4 : 2; }
+// matvec tiles of a trailing size with G row groups: group g holds spg (g + 1) strips
+__host__ __device__ constexpr int ntiles(int G, int spg) { return spg * G * (G + 1) / 2; }
 constexpr int NBMAX = 128;   // largest supported panel width
 constexpr int TAILN = 64;    // trailing size from which one workgroup finishes in LDS (k_tail)
 
@@ -102,47 +109,52 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
 // leaves its share of x^* M x (the lower part doubled off the diagonal) in
 // QF[b], from which k_wcol forms u^* M u without another grid-wide reduction.
 // Block (g, J) reads rows [s0 + SG g, +SG) x columns [s0 + SW J, +SW) (every
-// lower-triangle element exactly once over the grid); wave w of 8 takes 4 of
-// the 32 columns, one row per lane, all loads issued up front.  Row sums over
-// the 32 columns (summed across the waves in LDS) go to Prow[J][row]; column
-// sums over the 64 rows (a VALU-only halving butterfly, DPP and permlane
+// lower-triangle element exactly once over the grid); wave w of NWV takes SCW of
+// the SW columns, one row per lane, all loads issued up front.  Row sums over
+// the SW columns (summed across the waves in LDS) go to Prow[J][row]; column
+// sums over the SG rows (a VALU-only halving butterfly, DPP and permlane
 // swaps) go to Pcol[g][col]; then
 //   (M x)(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
 // every (slot, row) written exactly once, so the sum is deterministic.
-// Blocks are enumerated row-group major: group g has 2 (g + 1) strips.
-// The max(1, ceil(8 j / 64)) blocks after the G (G + 1) tiles compute the
+// Blocks are enumerated row-group major: group g has SPG (g + 1) strips.
+// The max(1, ceil(8 j / 64)) blocks after the ntiles(G) tiles compute the
 // reflector from the tail-norm partials (the first of them publishes gamma,
 // u1, x1 in scal and d[c], e[c]) and finish the tall-skinny products
 // TS = [W^* u | V^* u]: they add the per-block partials of W^* x and V^* x
-// that k_wcol left in TSP (8 waves over the partial index, then the 8
+// that k_wcol left in TSP (the waves over the partial index, then the
 // wave sums, in a fixed order), scale by gamma and add the row s0 term.
+template <int SYT, int SW>
 __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld, int n, int p, int c, int j,
                                               const quat* __restrict__ W, int ldw, const double* __restrict__ part,
                                               int nparts, double* __restrict__ scal, double* __restrict__ d,
                                               quat* __restrict__ e, quat* __restrict__ Prow, quat* __restrict__ Pcol,
                                               int ldp, quat* __restrict__ TS, int nb, const double* __restrict__ TSP,
                                               int ldtsp, double* __restrict__ QF, ZB zb) {
+  constexpr int NWV = SYT / 64;    // waves per block
+  constexpr int SCW = SW / NWV;     // strip columns per wave
+  constexpr int SPG = SG / SW;      // strips per row group and group index
+  static_assert(SW <= SG && SG % SW == 0 && SW % NWV == 0, "matvec tile shape");
   const int NT = j > 0 ? (8 * j + 63) / 64 : 0;
   const int NF = NT > 0 ? NT : 1;
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     nparts = (n - c + RQ - 1) / RQ;
     const int Gz = (n - c - 1 + SG - 1) / SG;
-    if ((int)blockIdx.x >= Gz * (Gz + 1) + NF) return;
+    if ((int)blockIdx.x >= ntiles(Gz, SPG) + NF) return;
     M = zoff(M, zb); W = zoff(W, zb); part = zoff(part, zb); scal = zoff(scal, zb); d = zoff(d, zb);
     e = zoff(e, zb); Prow = zoff(Prow, zb); Pcol = zoff(Pcol, zb); TS = zoff(TS, zb); TSP = zoff(TSP, zb);
     QF = zoff(QF, zb);
   }
   __shared__ double sh[SYT / 64];
   __shared__ double sqf[SYT / 64];
-  __shared__ double red[8][4][SG];   // wave, component, row: conflict-free b64 accesses
+  __shared__ double red[NWV][4][SG];   // wave, component, row: conflict-free b64 accesses
   const int s0 = c + 1;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: its column loads are scalar
   const int b = blockIdx.x;
   const int Gt = (n - s0 + SG - 1) / SG;
-  const int bt = b - Gt * (Gt + 1);
+  const int bt = b - ntiles(Gt, SPG);
   if (bt >= 0) {
     // ---- finishing block: reflector, scalars, tall-skinny products
     const bool has_ts = bt < NT;
@@ -163,11 +175,11 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     if (has_ts) {
       const int eidx = 64 * bt + lane;
       constexpr int TB = 16;
-      for (int k0 = w; k0 < nparts; k0 += 8 * TB) {
+      for (int k0 = w; k0 < nparts; k0 += NWV * TB) {
         double v[TB];
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
-          const int k = k0 + 8 * u;
+          const int k = k0 + NWV * u;
           v[u] = k < nparts ? TSP[(size_t)k * ldtsp + eidx] : 0.0;
         }
 #pragma unroll
@@ -190,7 +202,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
       qstore(e + c, alpha);
     }
     if (has_ts) {
-      __shared__ double tsr[8][64];
+      __shared__ double tsr[NWV][64];
       tsr[w][lane] = tsacc;
       __syncthreads();
       // quaternion (side, l) = entries 64 bt + 4 tid .. + 3, entry index 8 l + 4 side + component
@@ -200,7 +212,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
         for (int comp = 0; comp < 4; ++comp) {
           double t = 0.0;
 #pragma unroll
-          for (int k = 0; k < 8; ++k) t += tsr[k][4 * tid + comp];
+          for (int k = 0; k < NWV; ++k) t += tsr[k][4 * tid + comp];
           sc[comp] = t;
         }
         quat t = qscale(quat{sc[0], sc[1], sc[2], sc[3]}, R.g);
@@ -210,34 +222,34 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     }
     return;
   }
-  // ---- tile block: block -> (g, J) with g (g + 1) <= b < (g + 1) (g + 2)
-  int g = (int)((sqrt(4.0 * b + 1.0) - 1.0) * 0.5);
-  while ((g + 1) * (g + 2) <= b) ++g;
-  while (g * (g + 1) > b) --g;
-  const int J = b - g * (g + 1);
+  // ---- tile block: block -> (g, J) with ntiles(g) <= b < ntiles(g + 1)
+  int g = (int)((sqrt(8.0 * b / SPG + 1.0) - 1.0) * 0.5);
+  while (ntiles(g + 1, SPG) <= b) ++g;
+  while (ntiles(g, SPG) > b) --g;
+  const int J = b - ntiles(g, SPG);
   const int R0 = s0 + SG * g, C0 = s0 + SW * J;
   const bool active = C0 < n && R0 < n;   // uniform for the block
   const int ra = R0 + lane;
   const int rac = min(ra, n - 1);
-  quat qa[4];
+  quat qa[SCW];
 #pragma unroll
-  for (int cc = 0; cc < 4; ++cc) {
-    const int cgc = min(C0 + 4 * w + cc, n - 1);
+  for (int cc = 0; cc < SCW; ++cc) {
+    const int cgc = min(C0 + SCW * w + cc, n - 1);
     qa[cc] = qload(M + rac + (size_t)cgc * ld);
   }
   const quat xa = qload(M + rac + (size_t)c * ld);
   // x at this wave's 4 columns (the same address across the wave: one fetch each)
-  quat xc[4];
+  quat xc[SCW];
 #pragma unroll
-  for (int cc = 0; cc < 4; ++cc) xc[cc] = qload(M + min(C0 + 4 * w + cc, n - 1) + (size_t)c * ld);
+  for (int cc = 0; cc < SCW; ++cc) xc[cc] = qload(M + min(C0 + SCW * w + cc, n - 1) + (size_t)c * ld);
   const quat ua = (ra < n) ? xa : qzero();
   double qfl = 0.0;   // this thread's share of x^* M x over the tile
   if (active) {
     quat acc = qzero();
-    double col[16];
+    double col[4 * SCW];
 #pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-      const int cl = 4 * w + cc;
+    for (int cc = 0; cc < SCW; ++cc) {
+      const int cl = SCW * w + cc;
       const int cg = C0 + cl;
       quat a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
       if (ra == cg) {
@@ -255,11 +267,13 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     red[w][3][lane] = acc.bi;
     // x^* M x over the tile's lower part, doubled off the diagonal: 2 Re(conj(x_i) acc_i)
     qfl += 2.0 * (ua.ar * acc.ar + ua.ai * acc.ai + ua.br * acc.br + ua.bi * acc.bi);
-    // lane l ends with value (l >> 2) = component (l >> 2) & 3 of column (l >> 4), summed over the wave
-    const double cs = butterfly_sum<16>(col);
-    if ((lane & 3) == 0) {
-      const int cg = C0 + 4 * w + (lane >> 4);
-      if (cg < n) reinterpret_cast<double*>(Pcol + (size_t)g * ldp + cg)[(lane >> 2) & 3] = cs;
+    // lane l ends with value vi = l / LPV (component vi & 3 of column vi >> 2), summed over the wave
+    constexpr int LPV = 64 / (4 * SCW);
+    const double cs = butterfly_sum<4 * SCW>(col);
+    if (lane % LPV == 0) {
+      const int vi = lane / LPV;
+      const int cg = C0 + SCW * w + (vi >> 2);
+      if (cg < n) reinterpret_cast<double*>(Pcol + (size_t)g * ldp + cg)[vi & 3] = cs;
     }
   }
   qfl = wave_sum_d(qfl);
@@ -276,7 +290,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     if (i < n && i >= C0) {
       quat sum = qzero();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sum = qadd(sum, quat{red[k][0][tid], red[k][1][tid], red[k][2][tid], red[k][3][tid]});
+      for (int k = 0; k < NWV; ++k) sum = qadd(sum, quat{red[k][0][tid], red[k][1][tid], red[k][2][tid], red[k][3][tid]});
       qstore(Prow + (size_t)J * ldp + i, sum);
     }
   }
@@ -350,13 +364,15 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
   // predicated loads so that one wait covers them all (one batch up to
   // trailing sizes of ~4k rows with the wide variant, ~1k with the light one):
   //   y(i):  slots k = q + 16 t of row i's list, Prow[J][i] for J <= li / SW
+  //          (SW the strip width of this trailing size)
   //          (k < nJ), then Pcol[g][i] for g >= li / SG;
   //   x^* M x: QF[tid + RB t], t over the G (G + 1) tile partials;
   //   (M x)_{s0}: Prow[0][s0] and Pcol[g][s0] as slots tid + RB t of G + 1.
   const int li = ic - s0;
-  const int nJ = li / SW + 1, g0 = li / SG;
+  const int spg = matvec_spg(n - s0);
+  const int nJ = (spg == 4 ? li >> 4 : li >> 5) + 1, g0 = li / SG;
   const int cy = cdiv_d(nJ + G - g0 - q, 16);
-  const int nq = G * (G + 1);
+  const int nq = ntiles(G, spg);
   const int cq = cdiv_d(nq - tid, RB);
   const int cw0 = cdiv_d(G + 1 - tid, RB);
   quat yi = qzero(), w0 = qzero();
@@ -615,6 +631,12 @@ __global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* _
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+int matvec_row_slots(int nmax) { return std::max(cdiv(nmax, 32), cdiv(std::min(nmax, TSM), 16)); }
+int matvec_col_slots(int nmax) { return (nmax + SG - 1) / SG; }
+int matvec_tile_slots(int nmax) {
+  return std::max(ntiles(cdiv(nmax, SG) + 1, 2), ntiles(cdiv(std::min(nmax, TSM), SG) + 1, 4));
+}
+
 // panel columns per thread (16 threads per row) as a compile-time bound
 static int kpt_for(int nb) {
   const int k = cdiv(nb, 16);
@@ -663,12 +685,17 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
         hipLaunchKernelGGL(k_col_norm, dim3(cdiv(n - c, 256)), dim3(256), 0, s, M, ld, n, c, w.part, ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
+      const int spg = matvec_spg(sz);
+      const int nmain = ntiles(G, spg) + (j > 0 ? (8 * j + 63) / 64 : 1);
       {
         // algorithmic bytes: one read of the trailing lower triangle
         ProfScope ps(PROF_SYMV, s, 32.0 * 0.5 * (double)sz * (sz + 1));
-        launch_k(k_symv, dim3(nmain), dim3(SYT), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
-                 w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
+        if (spg == 4)
+          launch_k(k_symv<256, 16>, dim3(nmain), dim3(256), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
+                   w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
+        else
+          launch_k(k_symv<512, 32>, dim3(nmain), dim3(512), 0, s, M, ld, n, p, c, j, w.W, ldw, w.part, nblk,
+                   w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
       }
       launch_wcol(kpt, sz, dim3(cdiv(sz, RQ)), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal,
                   w.QF, w.part, w.TSP, w.ldtsp, ZB{nullptr, 0});
@@ -688,6 +715,13 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
   if (ts < n) hipLaunchKernelGGL(k_tail, dim3(1), dim3(RB), 0, s, M, ld, n, ts, nullptr, d, e, ZB{nullptr, 0});
   else hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
+// problem z's workspace: the batch base pointer advanced by z slabs
+template <class T>
+static T* zshift(T* p, size_t bytes) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(p) + bytes);
 }
 
 size_t reduce_batched_tasks(int nprob, const int* nh, int nb) {
@@ -748,10 +782,23 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     if (j == 0)
       hipLaunchKernelGGL(k_col_norm, dim3(cdiv(nrow, 256), 1, act), dim3(256), 0, s, M, ld, L, c, w.part, zb);
     const int sz = L - c - 1;
-    const int G = cdiv(sz, SG);
-    const int nmain = G * (G + 1) + (j > 0 ? (8 * j + 63) / 64 : 1);
-    hipLaunchKernelGGL(k_symv, dim3(nmain, 1, act), dim3(SYT), 0, s, M, ld, L, p, c, j, w.W, ldw, w.part, nblk,
-                       w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, w.QF, zb);
+    const int NF = j > 0 ? (8 * j + 63) / 64 : 1;
+    // each problem's matvec takes the tile shape of its own trailing size:
+    // problems [0, k1) wide strips, [k1, act) narrow ones (sizes descending)
+    int k1 = 0;
+    while (k1 < act && matvec_spg(nh[k1] - c - 1) == 2) ++k1;
+    if (k1 > 0)
+      hipLaunchKernelGGL((k_symv<512, 32>), dim3(ntiles(cdiv(sz, SG), 2) + NF, 1, k1), dim3(512), 0, s, M, ld, L,
+                         p, c, j, w.W, ldw, w.part, nblk, w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp,
+                         w.QF, zb);
+    if (k1 < act) {
+      const size_t sh = (size_t)k1 * zb.zs;
+      const int szn = nh[k1] - c - 1;
+      hipLaunchKernelGGL((k_symv<256, 16>), dim3(ntiles(cdiv(szn, SG), 4) + NF, 1, act - k1), dim3(256), 0, s,
+                         zshift(M, sh), ld, L, p, c, j, zshift(w.W, sh), ldw, zshift(w.part, sh), nblk,
+                         zshift(w.scal, sh), zshift(d, sh), zshift(e, sh), zshift(w.Prow, sh), zshift(w.Pcol, sh), ld,
+                         zshift(w.TS, sh), nb, zshift(w.TSP, sh), w.ldtsp, zshift(w.QF, sh), ZB{zb.nvec + k1, zb.zs});
+    }
     launch_wcol(kpt, sz, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
                 w.scal, w.QF, w.part, w.TSP, w.ldtsp, zb);
     if (j == nb - 1 || c == cend - 1) {

@@ -416,6 +416,12 @@ int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 // synthetic U(-1,1) lower triangle (real diagonal) for timing runs
 int gen_lower_uniform(int n, unsigned long long seed, quat* M, int ld, hipStream_t s);
+// partial-sum slots of the reduction's matvec for trailing sizes up to nmax:
+// row partials (per column strip), column partials (per row group), and the
+// per-tile x^* M x partials
+int matvec_row_slots(int nmax);
+int matvec_col_slots(int nmax);
+int matvec_tile_slots(int nmax);
 
 // gemm.hip: C = beta C + alpha opA(A) opB(B) for quaternion operands (FP64 MFMA).
 // An operand is the column concatenation [p0 | p1] split at ksplit along K;
