@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n2", type=int, default=8192)
     ap.add_argument("--nb", type=int, default=0,
-                    help="panel width (0: autotuned over 32/64/128 before the warm-up; the built-in table "
+                    help="panel width (0: autotuned over 16/32/64/128 before the warm-up; the built-in table "
                          "with --profile-run)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
@@ -432,7 +432,7 @@ def main():
     else:
         chosen, tms = ctx.autotune([n])
         nb = chosen[0]
-        tune = {str(w): round(float(t), 3) for w, t in zip((32, 64, 128), tms[0]) if t >= 0}
+        tune = {str(w): round(float(t), 3) for w, t in zip((16, 32, 64, 128), tms[0]) if t >= 0}
         print(f"autotune n={n}: reduction ms {tune} -> panel width {nb}", file=sys.stderr, flush=True)
 
     def step():

@@ -63,11 +63,11 @@ int zq_tuned_panel_width(int n);
 
 /* Panel-width autotuner (no counterpart in the reference, which hard-codes
  * nb = 20 at zquatev.cc:64).  For each half-dimension sizes[k] it times the
- * reduction, the one phase the panel width changes, with nb = 32, 64 and 128
+ * reduction, the one phase the panel width changes, with nb = 16, 32, 64 and 128
  * on a synthetic U(-1,1) matrix generated on the device, and records the
  * fastest for the size bucket floor(log2 n) of this context; later solves of
  * any n in that bucket use it unless zq_set_panel_width set an override.
- * chosen[k] (may be NULL) receives the pick; ms[3k + t] (may be NULL) the
+ * chosen[k] (may be NULL) receives the pick; ms[4k + t] (may be NULL) the
  * best of two timed runs in milliseconds for candidate t, -1 where the
  * candidate was skipped (wider than the matrix).  Synchronous; uses the
  * context's workspace.  Returns 0, or -2 (sizes NULL or an entry < 1), -3

@@ -1,6 +1,6 @@
 """Panel-width autotuner (zq_autotune): the reference hard-codes nb = 20
 (zquatev.cc:64); here the width is timed per size bucket on the device.  The
-pick must come from {32, 64, 128}, apply to every size of its bucket, yield
+pick must come from {16, 32, 64, 128}, apply to every size of its bucket, yield
 to an explicit override, and leave solves correct."""
 import sys
 
@@ -24,15 +24,15 @@ def test_autotune_picks_and_applies():
     with zq.Context(0) as ctx:
         chosen, ms = ctx.autotune([50, 700])
         # n = 50: 128 is wider than the matrix and skipped
-        assert chosen[0] in (32, 64) and ms[0, 2] == -1 and (ms[0, :2] > 0).all()
-        assert chosen[1] in (32, 64, 128) and (ms[1] > 0).all()
-        assert chosen[1] == (32, 64, 128)[int(np.argmin(ms[1]))]
+        assert chosen[0] in (16, 32, 64) and ms[0, 3] == -1 and (ms[0, :3] > 0).all()
+        assert chosen[1] in (16, 32, 64, 128) and (ms[1] > 0).all()
+        assert chosen[1] == (16, 32, 64, 128)[int(np.argmin(ms[1]))]
         # the pick covers the power-of-two bucket: [32, 64) and [512, 1024)
         assert ctx.panel_width(50) == chosen[0] and ctx.panel_width(33) == chosen[0]
         assert ctx.panel_width(700) == chosen[1] and ctx.panel_width(1000) == chosen[1]
         # an explicit width overrides the tuned one; 0 restores it
-        ctx.set_panel_width(16)
-        assert ctx.panel_width(700) == 16
+        ctx.set_panel_width(24)
+        assert ctx.panel_width(700) == 24
         ctx.set_panel_width(0)
         assert ctx.panel_width(700) == chosen[1]
         n = 700
@@ -49,4 +49,4 @@ def test_autotune_argument_errors():
         with pytest.raises(Exception):
             ctx.autotune([0])
         chosen, ms = ctx.autotune([])
-        assert chosen == [] and ms.shape == (0, 3)
+        assert chosen == [] and ms.shape == (0, 4)

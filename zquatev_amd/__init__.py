@@ -194,14 +194,14 @@ class Context:
         _check(_fn("zq_synchronize")(self._h), "zq_synchronize")
 
     def autotune(self, sizes):
-        """Time the reduction with panel widths 32, 64 and 128 for each
+        """Time the reduction with panel widths 16, 32, 64 and 128 for each
         half-dimension in ``sizes`` and keep the fastest for that size's
         power-of-two bucket on this context.  Returns ``(chosen, ms)``:
-        the pick per size and an array (len(sizes), 3) of the timings in ms
+        the pick per size and an array (len(sizes), 4) of the timings in ms
         (-1 where a width was skipped as wider than the matrix)."""
         sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
         chosen = np.zeros(max(sz.size, 1), dtype=np.int32)
-        ms = np.zeros((max(sz.size, 1), 3), dtype=np.float64)
+        ms = np.zeros((max(sz.size, 1), 4), dtype=np.float64)
         _check(_fn("zq_autotune")(self._h, sz.ctypes.data, int(sz.size), chosen.ctypes.data, ms.ctypes.data),
                "zq_autotune")
         return [int(v) for v in chosen[: sz.size]], ms[: sz.size]

