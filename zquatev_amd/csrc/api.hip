@@ -61,13 +61,20 @@ struct zq_context {
 namespace zq {
 static thread_local int g_pending = -1;   // slot armed by the innermost ProfScope
 
+// Events used only to time work on the device (phase marks, per-kernel
+// instrumentation, the autotuner): no system-scope fence when they are
+// recorded.  With the fence, a hipExtLaunchKernelGGL dispatch that carries
+// the stop event ends with a cache writeback; the measured 2n = 8192 matvec
+// then ran ~1.1 us (6 %) longer than without instrumentation.
+constexpr unsigned kTimingEvent = hipEventDisableSystemFence;
+
 ProfScope::ProfScope(int c, hipStream_t st, double u) : cls(c), s(st), units(u), slot(-1) {
   ProfState* p = g_prof;
   if (!p || !p->on) return;
   if (p->pool.size() < 2 * (p->used + 1)) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
+      if (hipEventCreateWithFlags(&e, kTimingEvent) != hipSuccess) return;
       p->pool.push_back(e);
     }
   }
@@ -783,7 +790,7 @@ int zq_create(zq_context** out, int device) {
   }
   c->stream = c->own_stream;
   for (auto& e : c->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
+    if (hipEventCreateWithFlags(&e, zq::kTimingEvent) != hipSuccess) {
       delete c;
       return ZQ_ERR_HIP;
     }
@@ -909,7 +916,7 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
   }
   static const int cand[4] = {16, 32, 64, 128};
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+  if (hipEventCreateWithFlags(&e0, zq::kTimingEvent) != hipSuccess || hipEventCreateWithFlags(&e1, zq::kTimingEvent) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
     return ZQ_ERR_HIP;
   }
