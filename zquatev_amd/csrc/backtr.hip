@@ -110,16 +110,18 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__
   const int wdt = min(WB, n - 1 - p);
   const quat* G = Gall + (size_t)blk * WB * WB;
   quat* T = Tall + (size_t)blk * WB * WB;
-  // Columns in groups of TB: thread r (its row of T) first applies the
-  // columns l < J of the rows it owns (one read of T(r, l) serves the group's
-  // TB columns, whose G entries are staged in LDS), then runs the recurrence
-  // inside the group from registers.  Every entry sums l = r .. jj-1 in
-  // increasing order, as the column-by-column recurrence does.  Blocks of 64
-  // keep T itself in LDS (its rows are re-read for every later group) and
-  // write it out at the end; blocks of 128 do not fit and use global memory.
-  constexpr int TB = 8;
+  // Columns in groups of TB = 256 / WB, one per wave set: thread (r, c) (its
+  // row r of T, column J + c of the group) first applies the columns l < J
+  // (T rows re-read from LDS or L2, the group's G entries staged in LDS);
+  // then the threads of c = 0 run the recurrence inside the group from
+  // those partials.  Every entry sums l = r .. jj-1 in increasing order, as
+  // the column-by-column recurrence does.  Blocks of 64 keep T itself in LDS
+  // (its rows are re-read for every later group) and write it out at the
+  // end; blocks of 128 do not fit and use global memory.
+  constexpr int TB = 256 / WB;
   constexpr bool INL = WB <= 64;
   __shared__ quat gs[TB][WB];
+  __shared__ quat ax[TB][WB];
   __shared__ quat Ts[INL ? WB : 1][INL ? WB : 1];   // Ts[col][row]
   auto tget = [&](int row, int col) -> quat {
     if constexpr (INL) return Ts[col][row];
@@ -129,35 +131,26 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__
     if constexpr (INL) Ts[col][row] = v;
     else qstore(T + row + (size_t)col * WB, v);
   };
-  const int r = threadIdx.x;
+  const int r = threadIdx.x % WB, cq = threadIdx.x / WB;
   for (int J = 0; J < WB; J += TB) {
+    const int jq = J + cq;
     if (J >= wdt) {   // columns past the block's reflectors: identity (uniform branch)
-      if (r < WB)
-#pragma unroll
-        for (int c = 0; c < TB; ++c) tset(r, J + c, r == J + c ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
+      tset(r, jq, r == jq ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
       continue;
     }
-    for (int idx = threadIdx.x; idx < TB * WB; idx += 256) {
-      const int c = idx / WB, l = idx - c * WB;
-      const int jj = J + c;
-      gs[c][l] = (jj < wdt && l < jj) ? qload(G + l + (size_t)jj * WB) : qzero();
-    }
+    gs[cq][r] = (jq < wdt && r < jq) ? qload(G + r + (size_t)jq * WB) : qzero();
     __syncthreads();
-    if (r < WB) {
-      quat acc[TB];
-#pragma unroll
-      for (int c = 0; c < TB; ++c) acc[c] = qzero();
-#pragma unroll 2
-      for (int l = r; l < J; ++l) {
-        const quat t = tget(r, l);
-#pragma unroll
-        for (int c = 0; c < TB; ++c) qfma(acc[c], t, gs[c][l]);
-      }
+    quat acc = qzero();
+#pragma unroll 4
+    for (int l = r; l < J; ++l) qfma(acc, tget(r, l), gs[cq][l]);
+    ax[cq][r] = acc;
+    __syncthreads();
+    if (cq == 0) {
       quat tb[TB];
 #pragma unroll
       for (int c = 0; c < TB; ++c) {
         const int jj = J + c;
-        quat a = acc[c];
+        quat a = ax[c][r];
 #pragma unroll
         for (int cc = 0; cc < c; ++cc)
           if (r <= J + cc) qfma(a, tb[cc], gs[c][J + cc]);
