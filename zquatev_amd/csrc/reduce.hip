@@ -117,7 +117,7 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
 //   (M x)(i) = sum_{J <= (i-s0)/SW} Prow[J][i] + sum_{g >= (i-s0)/SG} Pcol[g][i],
 // every (slot, row) written exactly once, so the sum is deterministic.
 // Blocks are enumerated row-group major: group g has SPG (g + 1) strips.
-// The max(1, ceil(8 j / 64)) blocks after the ntiles(G) tiles compute the
+// The max(1, ceil(8 j / 64)) blocks before the ntiles(G) tiles compute the
 // reflector from the tail-norm partials (the first of them publishes gamma,
 // u1, x1 in scal and d[c], e[c]) and finish the tall-skinny products
 // TS = [W^* u | V^* u]: they add the per-block partials of W^* x and V^* x
@@ -152,10 +152,11 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: its column loads are scalar
-  const int b = blockIdx.x;
-  const int Gt = (n - s0 + SG - 1) / SG;
-  const int bt = b - ntiles(Gt, SPG);
-  if (bt >= 0) {
+  // blocks [0, NF) finish the column (dispatched first, so they overlap the
+  // tiles), blocks NF + b take tile b
+  const int bt = blockIdx.x;
+  const int b = bt - NF;
+  if (b < 0) {
     // ---- finishing block: reflector, scalars, tall-skinny products
     const bool has_ts = bt < NT;
     // every load of the block is issued before the first wait: the row-s0
