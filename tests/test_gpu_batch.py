@@ -136,21 +136,26 @@ def test_batch_host_entry(gpu_ctx):
 
 def test_batch_host_entry_chunked(gpu_ctx):
     """Host batch whose copies fill several pinned chunks and split problems
-    between them; padding rows below a problem stay untouched."""
-    sizes = [900, 7, 260, 1000]
+    between them; padding rows below a problem stay untouched, and an empty
+    member (2n = 0) moves nothing."""
+    sizes = [900, 7, 0, 260, 1000]
     _say(f"case: host batch chunked {sizes}")
-    Ds = [gen.uniform_case(n, 7200 + n) for n in sizes]
+    Ds = [gen.uniform_case(n, 7200 + n) if n else None for n in sizes]
     work = []
     for k, (n, D) in enumerate(zip(sizes, Ds)):
         pad = 3 if k % 2 else 0
-        X = np.zeros((2 * n + pad, 2 * n), dtype=np.complex128, order="F")
-        X[: 2 * n, :] = D
+        X = np.zeros((max(2 * n + pad, 1), 2 * n), dtype=np.complex128, order="F")
+        if n:
+            X[: 2 * n, :] = D
         X[2 * n:, :] = 4.0 + 1.0j
         work.append(X)
     rc, info, eigs = gpu_ctx.zquatev_batched(work)
     assert rc == 0 and info == [0] * len(sizes)
     for k, n in enumerate(sizes):
         assert np.all(work[k][2 * n:, :] == 4.0 + 1.0j)
+        if n == 0:
+            assert eigs[k].size == 0
+            continue
         Xs1, e1 = _solo(gpu_ctx, Ds[k])
         assert np.array_equal(eigs[k], e1)
         assert np.array_equal(work[k][: 2 * n, :], Xs1)

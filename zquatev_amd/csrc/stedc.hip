@@ -447,7 +447,8 @@ __global__ void __launch_bounds__(256) k_merge_gather(const MergeDesc* __restric
 // at the midpoint of the interval), so t is small and the differences
 // delta_j - t are formed without cancellation.  Iteration: Newton on the
 // pole-free G(t) = (t - p_lo)(p_hi - t) f(t), safeguarded by a bisection
-// bracket, until the step falls below a few ulps of t.
+// bracket, until the step falls below a few ulps of t, starting from the root
+// of the two-pole model fitted at the midpoint.
 struct SecSums {
   double r, rp;   // rest = rho sum over the non-pole terms, and its derivative
 };
@@ -481,31 +482,51 @@ __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restri
   const int lane = threadIdx.x & 63;
   int org, plo, phi;
   double a, b, pl, ph;   // bracket (a, b) and pole positions relative to the origin
+  double t0;             // first iterate, relative to the origin
   if (i < K - 1) {
     const double gap = d[i + 1] - d[i];
     const double half = 0.5 * gap;
-    // f at the midpoint with origin d_i
-    double s = 0.0;
+    // f at the midpoint with origin d_i, and the part of it from the other poles
+    double s = 0.0, sr = 0.0;
     for (int jj = lane; jj < K; jj += 64) {
       const double zj = z[jj];
-      s += zj * zj / ((d[jj] - d[i]) - half);
+      const double q = zj * zj / ((d[jj] - d[i]) - half);
+      s += q;
+      if (jj != i && jj != i + 1) sr += q;
     }
     s = wave_sum_d(s);
+    sr = wave_sum_d(sr);
     plo = i;
     phi = i + 1;
     if (1.0 + rho * s >= 0.0) { org = i; a = 0.0; b = half; pl = 0.0; ph = gap; }
     else { org = i + 1; a = -half; b = 0.0; pl = -gap; ph = 0.0; }
+    // First iterate: the root in (0, gap) of the two-pole model
+    //   c + rho z_i^2 / (0 - t) + rho z_{i+1}^2 / (gap - t),  c = 1 + rho (other poles at the midpoint),
+    // i.e. of -c t^2 + (c gap + zl + zh) t - zl gap (the rational initial guess of
+    // LAPACK's dlaed4, in its simplest form).
+    const double zl = rho * z[i] * z[i], zh = rho * z[i + 1] * z[i + 1];
+    const double c = 1.0 + rho * sr;
+    const double a1 = c * gap + zl + zh, a0 = -zl * gap;
+    double r = -a0 / a1;
+    if (c != 0.0) {
+      const double disc = fmax(a1 * a1 + 4.0 * c * a0, 0.0);
+      const double qq = -0.5 * (a1 + copysign(sqrt(disc), a1));
+      const double r1 = qq / (-c), r2 = (qq != 0.0) ? a0 / qq : -1.0;
+      r = (r1 > 0.0 && r1 < gap) ? r1 : r2;
+    }
+    t0 = (org == i) ? r : r - gap;
   } else {
     double zz = 0.0;
     for (int jj = lane; jj < K; jj += 64) zz += z[jj] * z[jj];
     zz = wave_sum_d(zz);
     org = i; plo = i; phi = -1;
     a = 0.0; b = rho * zz; pl = 0.0; ph = 0.0;
+    t0 = 0.5 * (a + b);
   }
   const double dorg = d[org];
   const double zl2 = rho * z[plo] * z[plo];
   const double zh2 = (phi >= 0) ? rho * z[phi] * z[phi] : 0.0;
-  double t = 0.5 * (a + b);
+  double t = (t0 > a && t0 < b) ? t0 : 0.5 * (a + b);
   for (int it = 0; it < 200; ++it) {
     const SecSums S = sec_rest(d, z, K, dorg, rho, t, plo, phi);
     const double one = 1.0 + S.r;
