@@ -7,11 +7,12 @@
 // zq_zquatev_device takes them).  A scheduler thread takes whatever has
 // arrived and runs it as one batched call on a free lane: the oldest request
 // picks the batch's jobz and size class (half-dimension up to 192, up to 640,
-// up to 1280, larger: a lockstep batch runs as long as its largest member, so
-// requests are not held behind much larger ones), then further requests of
-// that jobz and class join in arrival order, up to max_batch problems and
-// max_bytes of workspace.  Batches of the two large classes occupy at most
-// all lanes but one, so small requests always find a lane soon.  Each lane has its own
+// up to 1024, up to 1450, larger: a lockstep batch runs as long as its
+// largest member, so requests are not held behind much larger ones), then
+// further requests of that jobz and class join in arrival order, up to
+// max_batch problems and max_bytes of workspace.  Batches of the large
+// classes (half-dimension above 640) occupy at most all lanes but one, so
+// small requests always find a lane soon.  Each lane has its own
 // thread that enqueues its batch (a large batch is thousands of launches),
 // so lanes never wait for each other's host work.  A
 // lane is a context with its own stream and resident workspace, so batches on
@@ -79,8 +80,8 @@ struct zq_server {
 
   double now_us() const { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
 
-  // classes 2 and 3 are the large ones
-  static int size_class(int n2) { return n2 <= 384 ? 0 : (n2 <= 1280 ? 1 : (n2 <= 2560 ? 2 : 3)); }
+  // classes 2 and above are the large ones
+  static int size_class(int n2) { return n2 <= 384 ? 0 : (n2 <= 1280 ? 1 : (n2 <= 2048 ? 2 : (n2 <= 2900 ? 3 : 4))); }
 
   int free_lane() const {
     for (size_t k = 0; k < lane_busy.size(); ++k)
