@@ -337,6 +337,9 @@ int splitk_count(int K, int nsplit, int* kchunk_out) {
 // ---------------------------------------------------------------------------
 constexpr int DBM = 64, DBN = 64, DBK = 16;
 
+// 64 x 64 tile per workgroup (2 x 2 waves of 32 x 32), K staged 16 at a time
+// through double-buffered LDS; the next stage's operands are fetched into
+// registers before the current stage's MFMAs (one barrier per stage).
 __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict__ descs, const int* __restrict__ kcnt,
                                                      const double* __restrict__ Qtmp, const double* __restrict__ U,
                                                      int lq, double* __restrict__ Z, int ldz,
@@ -351,40 +354,59 @@ __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict
   const int lda = m, ldb = m;
   double* C = Z + md.lo + (size_t)(md.lo - md.ob) * ldz;
   const int* cmap = posK + md.lo;
-  __shared__ double As[DBK][DBM + 1];
-  __shared__ double Bs[DBK][DBN + 1];
+  __shared__ double As[2][DBK][DBM + 1];
+  __shared__ double Bs[2][DBK][DBN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = wv >> 1, wc = wv & 1;  // 32 x 32 per wave
+  constexpr int NE = DBK * DBM / 256;   // stage elements per thread and operand
+  double ra[NE], rb[NE];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < NE; ++h) {
+      const int idx = tid + 256 * h;
+      const int i = idx & (DBM - 1), k = idx >> 6;
+      const int gi = row0 + i, gk = k0 + k;
+      ra[h] = (gi < m && gk < K) ? A[gi + (size_t)gk * lda] : 0.0;
+      const int kb = idx & (DBK - 1), jb = idx / DBK;
+      const int gkb = k0 + kb, gj = col0 + jb;
+      rb[h] = (gkb < K && gj < K) ? B[gkb + (size_t)gj * ldb] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < NE; ++h) {
+      const int idx = tid + 256 * h;
+      As[buf][idx >> 6][idx & (DBM - 1)] = ra[h];
+      Bs[buf][idx & (DBK - 1)][idx / DBK] = rb[h];
+    }
+  };
   d4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = 0; k0 < K; k0 += DBK) {
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int idx = tid + 256 * h;      // 1024 = 64 x 16
-      const int i = idx & (DBM - 1), k = idx >> 6;
-      const int gi = row0 + i, gk = k0 + k;
-      As[k][i] = (gi < m && gk < K) ? A[gi + (size_t)gk * lda] : 0.0;
-      const int kb = idx & (DBK - 1), jb = idx >> 4;
-      const int gkb = k0 + kb, gj = col0 + jb;
-      Bs[kb][jb] = (gkb < K && gj < K) ? B[gkb + (size_t)gj * ldb] : 0.0;
-    }
-    __syncthreads();
+    const bool more = k0 + DBK < K;
+    if (more) fetch(k0 + DBK);
 #pragma unroll
     for (int kk = 0; kk < DBK; kk += 4) {
       double af[2], bf[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[kk + (lane >> 4)][wr * 32 + 16 * a + (lane & 15)];
+      for (int a = 0; a < 2; ++a) af[a] = As[cur][kk + (lane >> 4)][wr * 32 + 16 * a + (lane & 15)];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[kk + (lane >> 4)][wc * 32 + 16 * b + (lane & 15)];
+      for (int b = 0; b < 2; ++b) bf[b] = Bs[cur][kk + (lane >> 4)][wc * 32 + 16 * b + (lane & 15)];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
+    if (more) stash(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
