@@ -556,7 +556,6 @@ __global__ void __launch_bounds__(256) k_tail(quat* __restrict__ M, int ld, int 
   __shared__ quat A[TAILN * TAILN];     // A[j * TAILN + i] = element (i, j)
   __shared__ quat us[TAILN], ys[TAILN];
   __shared__ quat part[RB / 64][TAILN];
-  __shared__ double sh[RB / 64];
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;   // row, column group
   for (int j = ty; j < ms; j += RB / 64) {
@@ -567,11 +566,21 @@ __global__ void __launch_bounds__(256) k_tail(quat* __restrict__ M, int ld, int 
     }
   }
   __syncthreads();
+  // Every row of the block lives in wave 0 (lane = row), so the two sums over
+  // rows per column (the tail norm, u^* y) are wave sums there, and the tail
+  // norm of the next column is formed by wave 0 right after it updates that
+  // column: four barriers per column.
+  __shared__ double tnorm;
+  if (ty == 0) {
+    const double v = (tx > 1 && tx < ms) ? qnorm2(A[tx]) : 0.0;
+    const double t = wave_sum_d(v);
+    if (tx == 0) tnorm = t;
+  }
+  __syncthreads();
   for (int cl = 0; cl + 1 < ms; ++cl) {
     const int r0 = cl + 1;                  // first row of the trailing block
     const bool row = tx >= r0 && tx < ms;   // this lane's row is in the trailing block
-    const double v = (ty == 0 && tx > r0 && tx < ms) ? qnorm2(A[cl * TAILN + tx]) : 0.0;
-    const double tail = block_sum(v, sh);
+    const double tail = tnorm;
     quat alpha;
     const Refl R = reflector(A[cl * TAILN + r0], tail, &alpha);
     if (tid == 0) {
@@ -589,17 +598,19 @@ __global__ void __launch_bounds__(256) k_tail(quat* __restrict__ M, int ld, int 
       for (int jj = r0 + ty; jj < ms; jj += RB / 64) qfma(acc, A[jj * TAILN + tx], us[jj]);
     part[ty][tx] = acc;
     __syncthreads();
-    double kv = 0.0;
-    if (ty == 0 && row) {
-      quat y = part[0][tx];
+    if (ty == 0) {
+      quat y = qzero();
+      double kv = 0.0;
+      if (row) {
+        y = part[0][tx];
 #pragma unroll
-      for (int k = 1; k < RB / 64; ++k) y = qadd(y, part[k][tx]);
-      ys[tx] = y;
-      const quat u = us[tx];
-      kv = u.ar * y.ar + u.ai * y.ai + u.br * y.br + u.bi * y.bi;   // Re(conj(u_i) y_i)
+        for (int k = 1; k < RB / 64; ++k) y = qadd(y, part[k][tx]);
+        const quat u = us[tx];
+        kv = u.ar * y.ar + u.ai * y.ai + u.br * y.br + u.bi * y.bi;   // Re(conj(u_i) y_i)
+      }
+      const double K = 0.5 * wave_sum_d(kv);
+      if (row) ys[tx] = qsub(y, qscale(us[tx], K));   // w
     }
-    const double K = 0.5 * block_sum(kv, sh);   // its barriers also publish ys
-    if (ty == 0 && row) ys[tx] = qsub(ys[tx], qscale(us[tx], K));   // w
     __syncthreads();
     if (row) {
       const quat ui = us[tx], wi = ys[tx];
@@ -610,6 +621,12 @@ __global__ void __launch_bounds__(256) k_tail(quat* __restrict__ M, int ld, int 
         if (jj == tx) a = quat{a.ar, 0.0, 0.0, 0.0};   // Hermitian diagonal: real
         A[jj * TAILN + tx] = a;
       }
+    }
+    // tail norm of the next column, r0, whose rows this wave (jj = r0) just updated
+    if (ty == 0) {
+      const double v = (tx > r0 + 1 && tx < ms) ? qnorm2(A[r0 * TAILN + tx]) : 0.0;
+      const double t = wave_sum_d(v);
+      if (tx == 0) tnorm = t;
     }
     __syncthreads();
   }
