@@ -16,6 +16,8 @@
 #include <complex>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -747,6 +749,18 @@ int d2h_runs(zq_context* c, const std::vector<ColRun>& runs, hipStream_t s) {
   return ZQ_OK;
 }
 
+int exception_status() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return ZQ_ERR_NOMEM;
+  } catch (const std::system_error&) {   // e.g. no thread could be started
+    return ZQ_ERR_NOMEM;
+  } catch (...) {
+    return ZQ_ERR_HIP;
+  }
+}
+
 extern "C" {
 
 int zq_version(void) { return 100; }
@@ -777,7 +791,7 @@ int zq_tuned_panel_width(int n) {
   return n <= 1024 ? 16 : 32;
 }
 
-int zq_create(zq_context** out, int device) {
+int zq_create(zq_context** out, int device) try {
   if (!out) return -1;
   *out = nullptr;
   int ndev = 0;
@@ -785,13 +799,13 @@ int zq_create(zq_context** out, int device) {
   zq_context* c = new zq_context();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
+    zq_destroy(c);
     return ZQ_ERR_HIP;
   }
   c->stream = c->own_stream;
   for (auto& e : c->ev) {
     if (hipEventCreateWithFlags(&e, zq::kTimingEvent) != hipSuccess) {
-      delete c;
+      zq_destroy(c);
       return ZQ_ERR_HIP;
     }
   }
@@ -806,6 +820,8 @@ int zq_create(zq_context** out, int device) {
   }
   *out = c;
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
 int zq_graph_reserve(zq_context* c, size_t bytes) {
@@ -900,7 +916,7 @@ int zq_panel_width(zq_context* c, int n) {
 // candidate width on a synthetic U(-1,1) input generated on the device: one
 // warm-up and two timed runs each, the faster kept.  Candidates wider than
 // the matrix are skipped (they would repeat the previous one's schedule).
-int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double* ms) {
+int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double* ms) try {
   if (!c) return ZQ_ERR_ARG;
   if (count < 0) return -3;
   if (count > 0 && !sizes) return -2;
@@ -955,9 +971,11 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
     if (chosen) chosen[k] = best;
   }
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
-int zq_zquatev_device(zq_context* c, int n2, double* dD, int nld2, double* deig, int jobz) {
+int zq_zquatev_device(zq_context* c, int n2, double* dD, int nld2, double* deig, int jobz) try {
   if (!c) return ZQ_ERR_ARG;
   int rc = check_args(n2, dD, nld2, deig);
   if (rc) return rc;
@@ -967,9 +985,11 @@ int zq_zquatev_device(zq_context* c, int n2, double* dD, int nld2, double* deig,
   CaptureScope cap(c);
   if (cap.rc) return cap.rc;
   return solve_device(c, n2 / 2, dD, nld2, deig, jobz == 'V' || jobz == 'v');
+} catch (...) {
+  return exception_status();
 }
 
-int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
+int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) try {
   if (!c) return ZQ_ERR_ARG;
   int rc = check_args(n2, D, nld2, eig);
   if (rc) return rc;
@@ -991,10 +1011,12 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) {
   if (c->info_dev) ZQ_HIP_CHECK(hipMemcpyAsync(&info, c->info_dev, sizeof(int), hipMemcpyDeviceToHost, s));
   ZQ_HIP_CHECK(hipStreamSynchronize(s));
   return info;
+} catch (...) {
+  return exception_status();
 }
 
 int zq_zquatev_batched_device(zq_context* c, int count, const int* n2, double* const* D, const int* ld2,
-                              double* const* eig, int jobz, int* info) {
+                              double* const* eig, int jobz, int* info) try {
   int rc = check_batch(c, count, n2, D, ld2, eig, info);
   if (rc) return rc;
   if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -7;
@@ -1004,10 +1026,12 @@ int zq_zquatev_batched_device(zq_context* c, int count, const int* n2, double* c
   if (cap.rc) return cap.rc;
   if (cap.active && info) return ZQ_ERR_GRAPH;   // info makes the call wait for the result
   return solve_batched(c, count, n2, D, ld2, eig, jobz == 'V' || jobz == 'v', info, nullptr);
+} catch (...) {
+  return exception_status();
 }
 
 int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
-                       int* info) {
+                       int* info) try {
   int rc = check_batch(c, count, n2, D, ld2, eig, info);
   if (rc) return rc;
   if (count == 0) return ZQ_OK;
@@ -1046,9 +1070,11 @@ int zq_zquatev_batched(zq_context* c, int count, const int* n2, double* const* D
   if (info)
     for (int i = 0; i < count; ++i) info[i] = inf[i];
   return solved;
+} catch (...) {
+  return exception_status();
 }
 
-int zq_zquatev(int n2, double* D, int nld2, double* eig) {
+int zq_zquatev(int n2, double* D, int nld2, double* eig) try {
   int rc = check_args(n2, D, nld2, eig);
   if (rc || n2 == 0) return rc;
   // one solve at a time on the shared context
@@ -1058,9 +1084,11 @@ int zq_zquatev(int n2, double* D, int nld2, double* eig) {
     if (rc) return rc;
   }
   return zq_zquatev_ctx(g_ctx, n2, D, nld2, eig);
+} catch (...) {
+  return exception_status();
 }
 
-int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double* dd, double* de) {
+int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double* dd, double* de) try {
   if (!c) return ZQ_ERR_ARG;
   int rc = check_args(n2, dD, nld2, dd);
   if (rc) return rc;
@@ -1078,9 +1106,11 @@ int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double*
   if (n > 1 && de)
     ZQ_HIP_CHECK(hipMemcpyAsync(de, p.ea, (size_t)(n - 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
-int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, double* dw, double* dZ) {
+int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, double* dw, double* dZ) try {
   if (!c) return ZQ_ERR_ARG;
   if (n < 0) return -2;
   if (n > 0 && (!dd || !dw)) return -3;
@@ -1099,6 +1129,8 @@ int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, dou
   if ((rc = zq::stage_end(c->stage, s))) return rc;
   if (vec) ZQ_HIP_CHECK(hipMemcpyAsync(dZ, p.sw.Z, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
 int zq_last_timings(zq_context* c, double* ms, int count) {

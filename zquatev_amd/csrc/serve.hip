@@ -153,17 +153,21 @@ struct zq_server {
       lane_has_job[L] = 0;
       lk.unlock();
       const int cnt = (int)b.reqs.size();
-      std::vector<int> n2s(cnt), ld2(cnt);
-      std::vector<double*> D(cnt), eig(cnt);
-      for (int k = 0; k < cnt; ++k) {
-        n2s[k] = b.reqs[k].n2;
-        ld2[k] = b.reqs[k].ld2;
-        D[k] = b.reqs[k].D;
-        eig[k] = b.reqs[k].eig;
-      }
       b.t_start = now_us();
-      b.rc = zq::batched_async(lanes[L], cnt, n2s.data(), D.data(), ld2.data(), eig.data(), b.reqs[0].jobz,
-                               lane_info[L]);
+      try {
+        std::vector<int> n2s(cnt), ld2(cnt);
+        std::vector<double*> D(cnt), eig(cnt);
+        for (int k = 0; k < cnt; ++k) {
+          n2s[k] = b.reqs[k].n2;
+          ld2[k] = b.reqs[k].ld2;
+          D[k] = b.reqs[k].D;
+          eig[k] = b.reqs[k].eig;
+        }
+        b.rc = zq::batched_async(lanes[L], cnt, n2s.data(), D.data(), ld2.data(), eig.data(), b.reqs[0].jobz,
+                                 lane_info[L]);
+      } catch (...) {   // the batch's requests complete with this status
+        b.rc = exception_status();
+      }
       if (hipEventRecord(lane_ev[L], zq::context_stream(lanes[L])) != hipSuccess && b.rc == 0) b.rc = ZQ_ERR_HIP;
       lk.lock();
       inflight.push_back(std::move(b));
@@ -223,7 +227,7 @@ struct zq_server {
 
 extern "C" {
 
-int zq_server_create(zq_server** out, int device, int lanes, int max_batch, double max_batch_gb) {
+int zq_server_create(zq_server** out, int device, int lanes, int max_batch, double max_batch_gb) try {
   if (!out) return ZQ_ERR_ARG;
   *out = nullptr;
   if (lanes < 1 || lanes > 8) return -3;
@@ -234,6 +238,19 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
   s->max_batch = max_batch;
   s->max_bytes = (size_t)(max_batch_gb * 1073741824.0);
   s->t0 = Clock::now();
+  try {   // the per-lane containers, sized once so that the loop below cannot throw
+    s->lanes.reserve(lanes);
+    s->lane_ev.reserve(lanes);
+    s->lane_info.reserve(lanes);
+    s->lane_busy.reserve(lanes);
+    s->lane_cls.reserve(lanes);
+    s->lane_has_job.reserve(lanes);
+    s->lane_job.reserve(lanes);
+    s->lane_threads.reserve(lanes);
+  } catch (...) {
+    delete s;
+    throw;
+  }
   int rc = ZQ_OK;
   for (int k = 0; k < lanes && rc == ZQ_OK; ++k) {
     zq_context* c = nullptr;
@@ -255,21 +272,43 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
     s->lane_has_job.push_back(0);
   }
   s->lane_job.resize(s->lanes.size());
-  if (rc) {
+  auto release = [s] {
     for (auto c : s->lanes) zq_destroy(c);
     for (auto e : s->lane_ev) (void)hipEventDestroy(e);
     for (auto h : s->lane_info) (void)hipHostFree(h);
     delete s;
+  };
+  if (rc) {
+    release();
     return rc;
   }
-  for (int k = 0; k < (int)s->lanes.size(); ++k) s->lane_threads.emplace_back([s, k] { s->run_lane(k); });
-  s->sched = std::thread([s] { s->run_scheduler(); });
-  s->compl_ = std::thread([s] { s->run_completion(); });
+  try {
+    for (int k = 0; k < (int)s->lanes.size(); ++k) s->lane_threads.emplace_back([s, k] { s->run_lane(k); });
+    s->sched = std::thread([s] { s->run_scheduler(); });
+    s->compl_ = std::thread([s] { s->run_completion(); });
+  } catch (...) {   // a thread could not be started: stop and join those that were
+    rc = exception_status();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_fly.notify_all();
+    s->cv_lane.notify_all();
+    for (auto& t : s->lane_threads)
+      if (t.joinable()) t.join();
+    if (s->sched.joinable()) s->sched.join();
+    if (s->compl_.joinable()) s->compl_.join();
+    release();
+    return rc;
+  }
   *out = s;
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
-int zq_server_submit(zq_server* s, int n2, double* dD, int ld2, double* deig, int jobz, unsigned long long* ticket) {
+int zq_server_submit(zq_server* s, int n2, double* dD, int ld2, double* deig, int jobz, unsigned long long* ticket) try {
   if (!s) return ZQ_ERR_ARG;
   if (n2 < 0 || (n2 & 1)) return -2;
   if (n2 > 0 && !dD) return -3;
@@ -288,6 +327,8 @@ int zq_server_submit(zq_server* s, int n2, double* dD, int ld2, double* deig, in
   ++s->outstanding;
   s->cv_work.notify_one();
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
 }
 
 int zq_server_poll(zq_server* s, zq_completion* out, int max) {

@@ -233,6 +233,11 @@ __device__ __forceinline__ quat wave_sum(quat v) {
     if (_e != hipSuccess) return (_e == hipErrorOutOfMemory) ? ZQI_ERR_NOMEM : ZQI_ERR_HIP; \
   } while (0)
 
+// Status for an exception caught at the C boundary (host containers, threads):
+// allocation and resource failures are ZQI_ERR_NOMEM, anything else
+// ZQI_ERR_HIP.  Called only from inside a catch handler.
+int exception_status() noexcept;
+
 namespace zq {
 
 // Optional per-kernel-class timing with HIP events (enabled per context).
