@@ -255,10 +255,11 @@ __global__ void __launch_bounds__(256, 2) k_qgemm2_batch(const QGemmTask* __rest
   qgemm2_tile<TBM, ACT, BCT>(t, row0, col0, t.kbeg, t.kend, t.C);
 }
 
-// Tile rows: 64 when the launch has enough 64 x 64 tiles to fill the device
-// (four per compute unit), else 32 (twice the workgroups for small products;
-// measured: the switch at four rather than two 64-row tiles per CU takes
-// ~0.5 ms off the 2n = 8192 reduction).
+// Tile rows: 64 only for launches with at least 16 64 x 64 tiles per compute
+// unit, else 32 (twice the workgroups: better balance and register use).
+// Measured on the 2n = 8192 solve (same box): the switch at 2 / 4 / 8 / 16
+// tiles per CU, or 32 rows always, gave 166.1 / 165.6 / 164.2 / 163.4 /
+// 163.3 ms; the batch of 512 problems is the same within 0.5%.
 static int device_cus() {
   static int cached[64] = {0};
   int dev = 0;
@@ -271,7 +272,7 @@ static int device_cus() {
   return cached[dev];
 }
 
-static int tile_rows(long tiles64) { return tiles64 >= 4L * device_cus() ? 64 : 32; }
+static int tile_rows(long tiles64) { return tiles64 >= 16L * device_cus() ? 64 : 32; }
 
 template <int TBM>
 static void launch_gemm2_t(bool act, bool bct, hipStream_t s, const QGemmTask& t, int nz, int kchunk,
