@@ -257,9 +257,10 @@ __global__ void __launch_bounds__(256, 2) k_qgemm2_batch(const QGemmTask* __rest
 
 // Tile rows: 64 only for launches with at least 16 64 x 64 tiles per compute
 // unit, else 32 (twice the workgroups: better balance and register use).
-// Measured on the 2n = 8192 solve (same box): the switch at 2 / 4 / 8 / 16
-// tiles per CU, or 32 rows always, gave 166.1 / 165.6 / 164.2 / 163.4 /
-// 163.3 ms; the batch of 512 problems is the same within 0.5%.
+// Measured on the 2n = 8192 solve, each step against the previous one on
+// one box: switch at 2 -> 4 tiles per CU -0.6 ms, 4 -> 8 -1.5 ms, 8 -> 16
+// -0.9 ms; 32 rows always is equal to 16 within noise; the batch of 512
+// problems stays within 0.5%.
 static int device_cus() {
   static int cached[64] = {0};
   int dev = 0;
