@@ -33,7 +33,7 @@ constexpr int SG = 64;       // matvec rows per block (one per lane)
 // small matvecs of batches and of the end of a solve spread over the CUs);
 // larger ones wide strips (SW = 32, 512 threads: fewer row partials for the
 // column kernel to gather).  spg = SG / SW strips per row group.
-constexpr int TSM = 512;
+constexpr int TSM = 768;
 __host__ __device__ constexpr int matvec_spg(int m) { return m <= TSM ? 4 : 2; }
 // matvec tiles of a trailing size with G row groups: group g holds spg (g + 1) strips
 __host__ __device__ constexpr int ntiles(int G, int spg) { return spg * G * (G + 1) / 2; }
@@ -662,13 +662,13 @@ static int kpt_for(int nb) {
 }
 
 // k_wcol variants: light gathers (more blocks per CU: batches of small
-// problems) up to trailing size 1024, wide ones (one gather batch) above.
+// problems) up to trailing size 1536, wide ones (one gather batch) above.
 template <int KPT, class... Args>
 static void launch_wcol_k(int sz, dim3 grid, hipStream_t s, Args... args) {
   // register budgets that compile without spills: 3 waves per SIMD at KPT 1,
   // 2 at KPT 2 and 4, 1 at KPT 8
   constexpr int WPE = KPT == 1 ? 3 : (KPT == 8 ? 1 : 2);
-  if (sz <= 1024) hipLaunchKernelGGL((k_wcol<KPT, 4, 2, WPE>), grid, dim3(RB), 0, s, args...);
+  if (sz <= 1536) hipLaunchKernelGGL((k_wcol<KPT, 4, 2, WPE>), grid, dim3(RB), 0, s, args...);
   else hipLaunchKernelGGL((k_wcol<KPT, 10, 18, 1>), grid, dim3(RB), 0, s, args...);
 }
 template <class... Args>
