@@ -302,7 +302,7 @@ class Server:
     """Continuous-batching server (zq_server_*): submit device-pointer requests,
     poll completion records."""
 
-    def __init__(self, device=0, lanes=3, max_batch=64, max_batch_gb=0.0):
+    def __init__(self, device=0, lanes=3, max_batch=128, max_batch_gb=0.0):
         h = _vp()
         _check(_fn("zq_server_create")(ctypes.byref(h), device, lanes, max_batch, float(max_batch_gb)),
                "zq_server_create")
