@@ -37,7 +37,16 @@ def parse():
                     help="panel width (0: autotuned over 16/32/64/128 before the warm-up; the built-in table "
                          "with --profile-run)")
     ap.add_argument("--seed", type=int, default=1001)
-    ap.add_argument("--cpu-n2", type=int, default=4096, help="size of the bounded CPU-baseline sample")
+    ap.add_argument("--cpu-n2", type=int, default=4096,
+                    help="size of the bounded CPU-baseline sample (median of --cpu-reps runs on the host cores)")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-small-reps", type=int, default=20,
+                    help="runs of the 2n=512 CPU workload (median), on all cores and on one")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving summary of the single workload")
+    ap.add_argument("--summary-serve-seconds", type=float, default=15.0)
+    ap.add_argument("--summary-loop-seconds", type=float, default=5.0)
+    ap.add_argument("--summary-capacity", type=int, default=1500)
+    ap.add_argument("--summary-pool", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-values-only", action="store_true", help="skip the eigenvalues-only (jobz='N') summary")
@@ -64,54 +73,102 @@ def parse():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-run: also run one elementwise copy (torch.neg) of MIB MiB, "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--cpu-child", nargs=2, type=int, metavar=("N2", "THREADS"), help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-child", nargs=3, type=int, metavar=("N2", "THREADS", "REPS"), help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
-def cpu_child(n2, threads):
-    """Runs in a subprocess with MKL_NUM_THREADS set: one reference solve."""
+def cpu_child(n2, threads, reps):
+    """Runs in a subprocess with MKL_NUM_THREADS set: `reps` timed reference
+    solves (wall clock around ts::zquatev only) of the same input."""
     import numpy as np
     from zquatev_amd import gen
     n = n2 // 2
-    D = gen.uniform_case(n, 1001)
+    D = gen.uniform_case(n, 1000 if n2 == 512 else 1001)   # the CPU workload's seed at 2n = 512
     kind = "reference"
     try:
         from oracle import ref
         if not ref.available():
             raise ImportError("oracle/_ref not built")
-        ref.zquatev(gen.uniform_case(8, 1))      # load MKL outside the timed call
-        t0 = time.perf_counter()
-        info, X, eig = ref.zquatev(D)
-        t = time.perf_counter() - t0
-        assert info == 0
+        ref.zquatev(gen.uniform_case(8, 1))      # load MKL outside the timed calls
+        solve = ref.zquatev
     except (ImportError, OSError):
         from oracle import zq_oracle
         kind = "port"
+        solve = zq_oracle.zquatev
+    ts = []
+    for _ in range(reps):
         t0 = time.perf_counter()
-        info, X, eig = zq_oracle.zquatev(D)
-        t = time.perf_counter() - t0
-    print(json.dumps({"seconds": t, "kind": kind, "n2": n2, "threads": threads,
+        info, X, eig = solve(D)
+        ts.append(time.perf_counter() - t0)
+        assert info == 0
+    print(json.dumps({"seconds": ts, "kind": kind, "n2": n2, "threads": threads,
                       "eig0": float(np.asarray(eig)[0])}))
 
 
-def cpu_baseline(n2):
-    threads = max(1, min(16, os.cpu_count() or 1))
+def _host_description():
+    """nproc / lscpu as the host reports them (the box may expose more cores
+    than its share)."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)"):
+                info["lscpu_" + k.strip().replace(" ", "_").replace("(s)", "s")] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _cpu_run(n2, threads, reps):
     env = dict(os.environ)
     env["MKL_NUM_THREADS"] = str(threads)
     env["OMP_NUM_THREADS"] = str(threads)
     env["HIP_VISIBLE_DEVICES"] = ""
-    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", str(n2), str(threads)],
-                         env=env, capture_output=True, text=True, timeout=900)
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", str(n2), str(threads), str(reps)],
+                         env=env, capture_output=True, text=True, timeout=1200)
     if out.returncode != 0:
-        return {"value": None, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
-                "sample": f"failed: {out.stderr[-300:]}"}
-    r = json.loads(out.stdout.strip().splitlines()[-1])
+        raise RuntimeError(out.stderr[-300:])
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(n2, reps, small_reps):
+    """The reference compiled from its own sources (oracle/_ref, MKL), timed on
+    this host: the median of `reps` solves at 2n = n2 on all the cores this
+    process may use (at most the 16 of one GPU's share of the box), plus the
+    2n = 512 CPU workload (median of `small_reps`) on those cores and on one."""
+    import numpy as np
     from zquatev_amd import gen
+    host = _host_description()
+    cores = host.get("affinity_cpus") or os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    try:
+        big = _cpu_run(n2, threads, reps)
+        small = _cpu_run(512, threads, small_reps)
+        small1 = _cpu_run(512, 1, small_reps)
+    except (RuntimeError, subprocess.SubprocessError, ValueError) as e:
+        return {"value": None, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+                "sample": f"failed: {e}", "host": host}
     n = n2 // 2
-    return {"value": round(gen.flops_vectors(n) / r["seconds"] / 1e9, 3), "unit": "GFLOP/s",
-            "cores": r["threads"], "kind": r["kind"],
-            "sample": (f"one ts::zquatev call (eigenvalues+eigenvectors) at 2n={n2}, same U(-1,1) generator, "
-                       f"{r['threads']} MKL threads, {r['seconds']:.2f} s; GFLOP/s on (160/3) n^3")}
+    med = float(np.median(big["seconds"]))
+    m512 = float(np.median(small["seconds"]))
+    m512_1 = float(np.median(small1["seconds"]))
+    f512 = gen.flops_vectors(256)
+    return {"value": round(gen.flops_vectors(n) / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": big["kind"],
+            "sample": (f"median of {reps} ts::zquatev calls (eigenvalues+eigenvectors) at 2n={n2} on {threads} MKL "
+                       f"threads, same U(-1,1) generator, wall clock around the call only "
+                       f"({', '.join(f'{t:.2f}' for t in big['seconds'])} s); GFLOP/s on (160/3) n^3"),
+            "host": host,
+            "runs_s": big["seconds"],
+            "cpu_workload_2n512": {
+                "seed": 1000, "runs": small_reps,
+                "median_ms_all_cores": round(1000 * m512, 3), "gflops_all_cores": round(f512 / m512 / 1e9, 2),
+                "median_ms_1_core": round(1000 * m512_1, 3), "gflops_1_core": round(f512 / m512_1 / 1e9, 2)}}
 
 
 def batched_workload(count, seed, dev):
@@ -172,12 +229,30 @@ def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
     t_loop = timed(loop_step, 1, 1)
     res_loop = [e.clone() for e in eigs[:8]]
     t_batch = timed(batch_step, steps, warmup)
+    # one instrumented batch: per-kernel-class times (HIP events on the library's stream)
+    ctx.set_instrument(True)
+    batch_step()
+    torch.cuda.synchronize()
+    kernels = {}
+    for cls in ctx.KERNEL_CLASSES:
+        t, k, u = ctx.instrument_report(cls)
+        v = {"launches": k, "total_ms": round(t, 3)}
+        if k and u and t > 0:
+            if cls == "symv":
+                v["GB_per_s"] = round(u / (t / 1000.0) / 1e9, 1)
+                v["frac_of_hbm_peak"] = round(u / (t / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+            else:
+                v["TFLOP_per_s"] = round(u / (t / 1000.0) / 1e12, 2)
+                v["frac_of_fp64_mfma_peak"] = round(u / (t / 1000.0) / 1e12 / FP64_MFMA_PEAK_TFS, 4)
+        kernels[cls] = v
+    ctx.set_instrument(False)
     out = {"problems": count, "n2_range": [min(n2s), max(n2s)], "seed": seed,
            "sum_n3": float(sum((v // 2) ** 3 for v in n2s)),
            "batched_solves_per_s": round(count / t_batch, 1), "batched_ms": round(1000 * t_batch, 2),
            "batched_gflops": round(flops / t_batch / 1e9, 1),
            "loop_solves_per_s": round(count / t_loop, 1), "loop_ms": round(1000 * t_loop, 2),
-           "speedup_vs_loop": round(t_loop / t_batch, 2)}
+           "speedup_vs_loop": round(t_loop / t_batch, 2),
+           "kernels": kernels}
     if check:
         # batch members equal the loop's solves (bitwise), and two members against H
         same = all(torch.equal(a, b) for a, b in zip(res_loop, eigs[:8]))
@@ -194,20 +269,25 @@ def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
     return out
 
 
-def run_serving(args):
+def run_serving(args, summary=False):
     """The serving workload: Poisson arrivals (rate args.rate) for
     args.serve_seconds; 2n log-uniform over even [128, 4096]; 70 % of the
     requests want eigenvectors.  Requests cycle through a pool of distinct
     Fock-like matrices resident in HBM (restored from a pristine copy before
     each use).  Served by the continuous-batching server, then by the plain
     loop of one synchronous device solve per request (FIFO) on the same
-    arrival times."""
+    arrival times.  summary=True: a bounded run (the --summary-* sizes) whose
+    figures are returned for the single workload's line instead of printed."""
     import numpy as np
     import torch
     import zquatev_amd as zq
     from zquatev_amd import gen
     dev = torch.device("cuda:0")
     T, lam, P = args.serve_seconds, args.rate, args.pool
+    loop_seconds, capacity_requests = args.loop_seconds, args.capacity_requests
+    if summary:
+        T, P = args.summary_serve_seconds, args.summary_pool
+        loop_seconds, capacity_requests = args.summary_loop_seconds, args.summary_capacity
     # arrivals and per-request choices (seed 1003)
     nmax_req = int(T * lam * 1.5) + 64
     gaps = -np.log1p(-gen.uniform(1003, np.arange(nmax_req))) / lam
@@ -304,8 +384,8 @@ def run_serving(args):
 
     # ---- saturation capacity: the same request mix, submitted back to back
     cap = None
-    if args.capacity_requests > 0:
-        ncap = args.capacity_requests
+    if capacity_requests > 0:
+        ncap = capacity_requests
         cdone, csub, last_cap = {}, {}, {}
         for r in range(ncap):
             s_ = r % P
@@ -331,7 +411,7 @@ def run_serving(args):
     # ---- plain loop: FIFO, one synchronous device solve per request, same arrivals
     loop_arr, loop_done = [], []
     t0 = time.perf_counter() + 0.001
-    limit = min(count, int(args.loop_seconds * lam)) if args.loop_seconds > 0 else count
+    limit = min(count, int(loop_seconds * lam)) if loop_seconds > 0 else count
     for r in range(limit):
         s_ = r % P
         due = t0 + arr[r]
@@ -348,6 +428,23 @@ def run_serving(args):
     loop = zq.latency_stats(loop_done, arrivals=loop_arr)
     loop["requests_offered"] = limit
     ctx.close()
+    speedup = round(serve["solves_per_s"] / loop["solves_per_s"], 2)
+    speedup_cap = round(cap["solves_per_s"] / loop["solves_per_s"], 2) if cap else None
+    if summary:
+        del work, pristine
+        return {"workload": f"Poisson arrivals {lam}/s for {T} s, 2n log-uniform over even [128, 4096], "
+                            "70% eigenvectors, Fock-like matrices",
+                "requests": count, "pool": P, "lanes": args.lanes, "max_batch": args.max_batch,
+                "p50_ms": round(serve["p50_ms"], 3), "p99_ms": round(serve["p99_ms"], 3),
+                "solves_per_s": round(serve["solves_per_s"], 1), "mean_batch": round(serve["mean_batch"], 2),
+                "check": serve["check"], "bad_info": serve["bad_info"],
+                "plain_loop": {"requests": loop["completed"], "p50_ms": round(loop["p50_ms"], 3),
+                               "p99_ms": round(loop["p99_ms"], 3), "solves_per_s": round(loop["solves_per_s"], 1)},
+                "speedup_offered_rate": speedup,
+                "capacity": cap, "speedup_capacity": speedup_cap,
+                "note": ("at the offered rate the server's solves/s is the arrival rate (it keeps up), so "
+                         "speedup_offered_rate is bounded by offered rate / plain-loop capacity; "
+                         "speedup_capacity compares both saturated")}
     line = {"metric": "serving solves/s + p50/p99 latency vs plain loop", "value": round(serve["solves_per_s"], 1),
             "unit": "solves/s", "n_gpus": 1, "steps": count, "warmup": len(big),
             "ms_per_step": round(1000.0 / serve["solves_per_s"], 3), "higher_is_better": True, "scaling": "weak",
@@ -360,9 +457,9 @@ def run_serving(args):
             # at the offered load the server's solves/s is the arrival rate (it keeps up), so
             # this ratio is bounded by offered / loop capacity; the capacity ratio compares the
             # two saturated (the loop is saturated at this load: its solves/s is its capacity)
-            "speedup_solves_per_s": round(serve["solves_per_s"] / loop["solves_per_s"], 2),
+            "speedup_solves_per_s": speedup,
             "capacity": cap,
-            "speedup_capacity": round(cap["solves_per_s"] / loop["solves_per_s"], 2) if cap else None}
+            "speedup_capacity": speedup_cap}
     print(json.dumps(line))
     return 0
 
@@ -575,9 +672,17 @@ def main():
         torch.cuda.empty_cache()
         batched = run_batched(ctx, stream, args.batch_count, 1002, 2, 1, check=not args.no_check)
 
+    serving = None
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        try:
+            serving = run_serving(args, summary=True)
+        except Exception as e:   # the single-solve line stands on its own; record what failed
+            serving = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_n2)
+        cpu = cpu_baseline(args.cpu_n2, args.cpu_reps, args.cpu_small_reps)
 
     line = {
         "metric": METRIC,
@@ -604,6 +709,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "batched": batched,
+        "serving": serving,
         "values_only": vals,
         "host_entry": host,
     }

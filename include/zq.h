@@ -16,7 +16,10 @@
  * elements, exactly as std::complex<double>* in the reference.
  *
  * Return codes: 0 success; -i argument i is invalid (LAPACK style); > 0 the
- * tridiagonal eigensolver did not converge; ZQ_ERR_* below for runtime errors.
+ * tridiagonal eigensolver did not converge (as zhbev's info at zquatev.cc:84,
+ * :99: a leaf QL iteration, a secular-equation root or an eigenvalue bracket
+ * that did not converge; the value is 1 + a local row or root index);
+ * ZQ_ERR_* below for runtime errors and for a non-finite input.
  */
 #ifndef ZQ_H
 #define ZQ_H
@@ -32,6 +35,10 @@ extern "C" {
 #define ZQ_ERR_NOMEM (-1002)  /* device memory exhausted */
 #define ZQ_ERR_ARG (-1003)    /* unsupported internal configuration */
 #define ZQ_ERR_GRAPH (-1004)  /* graph capture: a step that cannot be captured, or resources held by graphs */
+#define ZQ_ERR_NONFINITE (-1005)  /* the input holds a NaN or an infinity (or its magnitude overflowed the
+                                     reduction); eigenvalues are returned as NaN, vectors are unspecified.  An
+                                     addition of this build: the reference does not check its input. */
+#define ZQ_ERR_BUDGET (-1006) /* serving: the request alone needs more device memory than the server's budget */
 
 typedef struct zq_context zq_context;
 
@@ -61,21 +68,38 @@ int zq_set_panel_width(zq_context* ctx, int nb);
 /* Panel width the built-in table picks for half-dimension n. */
 int zq_tuned_panel_width(int n);
 
-/* Panel-width autotuner (no counterpart in the reference, which hard-codes
- * nb = 20 at zquatev.cc:64).  For each half-dimension sizes[k] it times the
- * reduction, the one phase the panel width changes, with nb = 16, 32, 64 and 128
+/* Panel-width and tile-shape autotuner (no counterpart in the reference,
+ * which hard-codes nb = 20 at zquatev.cc:64).  For each half-dimension
+ * sizes[k] it times the reduction, the phase the panel width changes, with
+ * nb = 16, 32, 64 and 128
  * on a synthetic U(-1,1) matrix generated on the device, and records the
  * fastest for the size bucket floor(log2 n) of this context; later solves of
  * any n in that bucket use it unless zq_set_panel_width set an override.
  * chosen[k] (may be NULL) receives the pick; ms[4k + t] (may be NULL) the
  * best of two timed runs in milliseconds for candidate t, -1 where the
- * candidate was skipped (wider than the matrix).  Synchronous; uses the
- * context's workspace.  Returns 0, or -2 (sizes NULL or an entry < 1), -3
+ * candidate was skipped (wider than the matrix).  With the pick it then
+ * tunes the bucket's tile shapes (zq_set_tile_shape below).  Synchronous;
+ * uses the context's workspace.  Returns 0, or -2 (sizes NULL or an entry < 1), -3
  * (count < 0), or a ZQ_ERR_* code. */
 int zq_autotune(zq_context* ctx, const int* sizes, int count, int* chosen, double* ms);
 
 /* Panel width a solve of half-dimension n on this context uses now. */
 int zq_panel_width(zq_context* ctx, int n);
+
+/* Tile shapes (no counterpart in the reference, whose blocking is fixed at
+ * nb = 20, zquatev.cc:64).  zq_autotune also times, per size bucket and
+ * after the panel width, the panel matvec's strip switch -- narrow column
+ * strips (16 quaternions) for trailing sizes up to this many rows, wide (32)
+ * above; candidates 512, 768, 1024, 1536; default 768 -- and the quaternion
+ * GEMM's tile rows -- 64-row tiles for launches of at least this many
+ * 64 x 64 tiles per compute unit, else 32-row tiles; candidates 4, 16, 64;
+ * default 16.  zq_set_tile_shape overrides both (switch -1 and threshold 0
+ * restore the tuned values; switch 0..2048, -2 / -3 when out of range).
+ * The switch changes the matvec's summation order, so a batch groups its
+ * problems by it (as by panel width); the tile rows change no result.
+ * zq_tile_shape reports what a solve of half-dimension n uses now. */
+int zq_set_tile_shape(zq_context* ctx, int matvec_switch_rows, int gemm64_tiles_per_cu);
+int zq_tile_shape(zq_context* ctx, int n, int* matvec_switch_rows, int* gemm64_tiles_per_cu);
 
 /* HIP graph capture (no counterpart in the reference, a host library with no
  * streams).  zq_zquatev_device and zq_zquatev_batched_device (with info =
@@ -110,7 +134,8 @@ int zq_zquatev_ctx(zq_context* ctx, int n2, double* D, int nld2, double* eig);
 /* Device-memory variant: dD (device, nld2 x n2 complex128), deig (device,
  * n2/2 doubles).  jobz = 'V' computes eigenvectors into dD as above; jobz =
  * 'N' computes eigenvalues only and leaves dD unchanged.  Asynchronous on the
- * context's stream. */
+ * context's stream: the return value covers argument checks and launches,
+ * and zq_last_info gives the solve's status (0, > 0, ZQ_ERR_NONFINITE). */
 int zq_zquatev_device(zq_context* ctx, int n2, double* dD, int nld2, double* deig, int jobz);
 
 /* Batched solves of independent problems of uneven sizes in one call.  The
@@ -126,7 +151,11 @@ int zq_zquatev_device(zq_context* ctx, int n2, double* dD, int nld2, double* dei
  * Returns 0; ZQ_ERR_ARG for a null context; -2 count < 0; -3..-6 a null n2,
  * D, ld2 or eig array; -7 bad jobz; -8 some problem's own arguments are
  * invalid (nothing is run; info[i] holds that problem's zq_zquatev code,
- * -1..-4); 1 if some problem did not converge; ZQ_ERR_HIP / ZQ_ERR_NOMEM. */
+ * -1..-4); ZQ_ERR_NONFINITE if some problem's input held a NaN or an
+ * infinity, else 1 if some problem did not converge (info[i] holds each
+ * problem's own code: 0, > 0 or ZQ_ERR_NONFINITE; the other problems are
+ * solved normally); ZQ_ERR_HIP / ZQ_ERR_NOMEM.  Without info the call is
+ * asynchronous and zq_last_info reports the aggregate. */
 int zq_zquatev_batched_device(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2,
                               double* const* eig, int jobz, int* info);
 
@@ -145,14 +174,26 @@ int zq_zquatev_batched_device(zq_context* ctx, int count, const int* n2, double*
 typedef struct zq_server zq_server;
 typedef struct {
   unsigned long long ticket;
-  int info;         /* 0; > 0 no convergence; < 0 error code of the batched call */
+  int info;         /* 0; > 0 no convergence; ZQ_ERR_NONFINITE, ZQ_ERR_BUDGET, or another error code */
   int batch_size;   /* problems in the batch that served the request */
   int lane;
   double t_submit_us, t_start_us, t_done_us;
 } zq_completion;
 
-/* lanes 1..8 (-3), max_batch >= 1 (-4), max_batch_gb >= 0, 0 = no limit (-5). */
+/* lanes 1..8 (-3), max_batch >= 1 (-4), max_batch_gb >= 0, 0 = no limit (-5).
+ * The lanes' workspaces together stay within a device-memory budget, by
+ * default 80% of the device memory free at creation (zq_server_set_budget).
+ * A lane's workspace grows only while no other lane has work in flight (a
+ * reallocation waits for the whole device): a batch is cut to what its lane
+ * holds while others run, and a request that needs more waits for them.  A
+ * request that alone needs more than the budget completes at once with info
+ * ZQ_ERR_BUDGET; a request whose input holds a NaN or an infinity completes
+ * with ZQ_ERR_NONFINITE.  Neither affects the other requests. */
 int zq_server_create(zq_server** out, int device, int lanes, int max_batch, double max_batch_gb);
+/* Device-memory budget of the lanes' workspaces in GiB (> 0), or 0 for the
+ * default; applies to batches scheduled from now on (-2: negative). */
+int zq_server_set_budget(zq_server* srv, double budget_gb);
+double zq_server_budget_gb(zq_server* srv);
 /* Queue one request; arguments as zq_zquatev_device (errors -2..-6 for n2,
  * dD, ld2, deig, jobz).  Returns at once with its ticket. */
 int zq_server_submit(zq_server* srv, int n2, double* dD, int ld2, double* deig, int jobz,
@@ -209,6 +250,38 @@ int zq_selftest_mfma(zq_context* ctx, const double* dA, const double* dB, double
 
 /* Stream synchronisation helper for callers without their own HIP runtime binding. */
 int zq_synchronize(zq_context* ctx);
+
+/* Status of the last solve issued on ctx by an asynchronous entry point
+ * (zq_zquatev_device, zq_tridiagonalize, zq_tridiag_eig, or a batched call):
+ * waits for the context's stream, then returns 0, ZQ_ERR_NONFINITE, or > 0
+ * for no convergence -- the code zhbev's info carries in the reference
+ * (zquatev.cc:84, :99), which the host entry points return directly.  For a
+ * batch, the aggregate its batched call returns. */
+int zq_last_info(zq_context* ctx);
+
+/* Iteration caps of the tridiagonal eigensolver on ctx (0 = default): QL
+ * sweeps per eigenvalue of a divide & conquer leaf (60), safeguarded Newton
+ * steps per secular-equation root (200), multisection steps per eigenvalue
+ * of the eigenvalues-only path (64).  A cap reached is reported as
+ * non-convergence (info > 0), as zhbev reports a QL failure (zquatev.cc:84).
+ * Meant for tests of that path; -2 for a negative cap. */
+int zq_set_iteration_caps(zq_context* ctx, int leaf, int secular, int bisect);
+
+/* Cap the device workspace ctx may hold (bytes; 0 = no cap but free device
+ * memory).  A batch whose workspace exceeds the cap, or what the device can
+ * give, is solved as consecutive runs of its size-sorted problems, each run
+ * fitting (results unchanged: each member still equals its solo solve); a
+ * single problem over the cap returns ZQ_ERR_NOMEM.  No counterpart in the
+ * reference, which allocates per call (zquatev.cc:63-66). */
+int zq_set_workspace_limit(zq_context* ctx, size_t bytes);
+
+/* Device workspace bytes a batched call of these sizes (jobz 'V' or 'N')
+ * needs on ctx (NULL: the built-in panel-width table), split into runs that
+ * fit `budget` bytes (0: one run per panel-width group).  *bytes receives
+ * the largest run's workspace, run[i] (optional, count ints) problem i's run
+ * (-1 for an empty problem).  Returns the number of runs, or -2 count < 0,
+ * -3 bad sizes, -4 bad jobz.  Host arithmetic only: no device call. */
+int zq_batch_workspace(zq_context* ctx, int count, const int* n2, int jobz, size_t budget, size_t* bytes, int* run);
 
 #ifdef __cplusplus
 }

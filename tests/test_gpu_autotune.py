@@ -50,3 +50,48 @@ def test_autotune_argument_errors():
             ctx.autotune([0])
         chosen, ms = ctx.autotune([])
         assert chosen == [] and ms.shape == (0, 4)
+
+
+def test_autotune_tile_shapes_and_overrides():
+    """zq_autotune also picks the matvec strip switch and the GEMM tile-rows
+    threshold for the bucket; overrides take precedence; every switch gives a
+    correct solve, and a batch equals its solo solves bit for bit under a
+    non-default switch (the batch groups problems by it)."""
+    import torch
+    _say("case: tile-shape autotune 1500")
+    with zq.Context(0) as ctx:
+        assert ctx.tile_shape(1500) == (768, 16)           # defaults before tuning
+        ctx.autotune([1500])
+        tsm, t64 = ctx.tile_shape(1500)
+        assert tsm in (512, 768, 1024, 1536) and t64 in (4, 16, 64)
+        assert ctx.tile_shape(1100) == (tsm, t64)            # the same bucket [1024, 2048)
+        for sw in (0, 300, 2048):
+            ctx.set_tile_shape(sw, 4)
+            assert ctx.tile_shape(1500) == (sw, 4)
+            n = 500
+            D = gen.uniform_case(n, 7100 + sw)
+            work = np.asfortranarray(D.copy())
+            info, eig = ctx.zquatev(work)
+            assert info == 0
+            inv.assert_solution(gen.full_hermitian(D, n), work, eig)
+        ctx.set_tile_shape(300, 0)
+        sizes = [500, 120, 380]
+        Ds = [gen.uniform_case(n, 7200 + k) for k, n in enumerate(sizes)]
+        dev = [torch.from_numpy(np.ascontiguousarray(D.T)).cuda() for D in Ds]
+        eig = [torch.zeros(n, dtype=torch.float64, device="cuda") for n in sizes]
+        torch.cuda.synchronize()
+        rc, info = ctx.zquatev_batched_device([2 * n for n in sizes], [t.data_ptr() for t in dev],
+                                              [2 * n for n in sizes], [e.data_ptr() for e in eig], "V",
+                                              want_info=True)
+        assert rc == 0 and info == [0, 0, 0]
+        for k, (n, D) in enumerate(zip(sizes, Ds)):
+            t = torch.from_numpy(np.ascontiguousarray(D.T)).cuda()
+            e = torch.zeros(n, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            assert ctx.zquatev_device(2 * n, t.data_ptr(), 2 * n, e.data_ptr(), "V") == 0
+            ctx.synchronize()
+            assert torch.equal(e, eig[k]) and torch.equal(t, dev[k]), f"member {k}"
+        ctx.set_tile_shape(-1, 0)
+        assert ctx.tile_shape(1500) == (tsm, t64)
+        with pytest.raises(zq.ZQError):
+            ctx.set_tile_shape(4096, 0)

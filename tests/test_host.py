@@ -132,3 +132,58 @@ def test_server_validates_without_device():
     assert L.zq_server_submit(None, 4, None, 4, None, ord("V"), ctypes.byref(t)) == zq.ERR_ARG
     assert L.zq_server_poll(None, None, 0) == zq.ERR_ARG
     assert L.zq_server_destroy(None) == 0
+
+
+def test_explicit_n2_must_fit_D():
+    D = np.zeros((4, 4), dtype=np.complex128, order="F")
+    with pytest.raises(ValueError):
+        zq.zquatev(D, n2=6)            # more columns than D has: would overflow on the host
+    with pytest.raises(ValueError):
+        zq.zquatev(np.zeros((4, 6), dtype=np.complex128, order="F"), n2=6)   # ld2 < n2
+
+
+def test_new_codes_and_setters_validate_without_device():
+    L = zq.lib()
+    assert "NaN" in zq.error_string(zq.ERR_NONFINITE)
+    assert "budget" in zq.error_string(zq.ERR_BUDGET)
+    assert L.zq_last_info(None) == zq.ERR_ARG
+    assert L.zq_set_workspace_limit(None, 0) == zq.ERR_ARG
+    assert L.zq_set_iteration_caps(None, 1, 1, 1) == zq.ERR_ARG
+    assert L.zq_server_set_budget(None, ctypes.c_double(1.0)) == zq.ERR_ARG
+
+
+def test_batch_workspace_split_plan():
+    """A heavy-tailed batch (one 2n = 8192 problem and many of 2n = 2100 in one
+    panel-width group) needs a slab sized for the largest problem per member;
+    split to a budget, it becomes consecutive runs of the size-sorted problems,
+    each within the budget, covering every problem once."""
+    sizes = [2100] * 60 + [8192] + [2100] * 60
+    whole, runs, _ = zq.batch_workspace(sizes, "V")
+    assert runs == 1
+    budget = 40 << 30
+    assert whole > budget
+    need, runs, run = zq.batch_workspace(sizes, "V", budget=budget)
+    assert runs >= 2 and need <= budget
+    assert run[60] == 0                                   # the largest problem leads the first run
+    assert sorted(set(run)) == list(range(runs))
+    for r in range(runs):
+        members = [s for s, q in zip(sizes, run) if q == r]
+        got, k, _ = zq.batch_workspace(members, "V")
+        assert k == 1 and got <= budget
+    # consecutive in descending size order: no run holds a larger problem than an earlier run
+    firsts = [max(s for s, q in zip(sizes, run) if q == r) for r in range(runs)]
+    assert firsts == sorted(firsts, reverse=True)
+    # values only needs less than vectors; a problem larger than the budget forms its own run
+    assert zq.batch_workspace(sizes, "N")[0] < whole
+    need1, _, run1 = zq.batch_workspace([8192, 8192, 64], "V", budget=1 << 20)
+    assert len(set(run1)) == 3 and need1 > (1 << 20)
+    assert zq.batch_workspace([0, 64], "V")[2][0] == -1   # empty problems belong to no run
+    with pytest.raises(zq.ZQError):
+        zq.batch_workspace([63], "V")
+
+
+def test_dropin_caller_is_built():
+    exe = os.path.join(ROOT, "tests", "dropin", "zquatev_caller")
+    assert os.path.exists(exe), "built by zquatev_amd/csrc/Makefile (__graft_entry__.build)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

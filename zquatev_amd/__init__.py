@@ -32,6 +32,8 @@ _SIGNATURES = {
     "zq_tuned_panel_width": (_c_int, [_c_int]),
     "zq_autotune": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
     "zq_panel_width": (_c_int, [_vp, _c_int]),
+    "zq_set_tile_shape": (_c_int, [_vp, _c_int, _c_int]),
+    "zq_tile_shape": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "zq_graph_reserve": (_c_int, [_vp, ctypes.c_size_t]),
     "zq_graph_release": (_c_int, [_vp]),
     "zq_zquatev": (_c_int, [_c_int, _vp, _c_int, _vp]),
@@ -50,6 +52,12 @@ _SIGNATURES = {
     "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
     "zq_selftest_mfma": (_c_int, [_vp, _vp, _vp, _vp]),
     "zq_synchronize": (_c_int, [_vp]),
+    "zq_last_info": (_c_int, [_vp]),
+    "zq_set_workspace_limit": (_c_int, [_vp, ctypes.c_size_t]),
+    "zq_set_iteration_caps": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "zq_batch_workspace": (_c_int, [_vp, _c_int, _vp, _c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _vp]),
+    "zq_server_set_budget": (_c_int, [_vp, ctypes.c_double]),
+    "zq_server_budget_gb": (ctypes.c_double, [_vp]),
     "zq_set_instrument": (_c_int, [_vp, _c_int]),
     "zq_instrument_report": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_c_int),
                                       ctypes.POINTER(ctypes.c_double)]),
@@ -99,6 +107,8 @@ ERR_HIP = -1001
 ERR_NOMEM = -1002
 ERR_ARG = -1003
 ERR_GRAPH = -1004
+ERR_NONFINITE = -1005
+ERR_BUDGET = -1006
 
 
 class ZQError(RuntimeError):
@@ -133,14 +143,40 @@ def _fortran_c128(D):
     return D
 
 
+def _host_n2(D, n2):
+    """The order n2 solved in D: all of D's columns by default; an explicit n2
+    must fit D (the C entry writes n2 columns of D.shape[0] rows)."""
+    if n2 is None:
+        return D.shape[1]
+    n2 = int(n2)
+    if n2 > D.shape[1] or n2 > D.shape[0]:
+        raise ValueError(f"n2={n2} does not fit D of shape {D.shape}")
+    return n2
+
+
+def batch_workspace(sizes, jobz="V", budget=0, ctx=None):
+    """Device workspace (bytes) a batched call of these orders 2n needs, and
+    each problem's run when the batch is split to fit `budget` bytes (0: no
+    split).  Host arithmetic only (no device call).  Returns (bytes, runs,
+    run index per problem)."""
+    sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
+    run = np.zeros(max(sz.size, 1), dtype=np.int32)
+    nbytes = ctypes.c_size_t()
+    h = ctx.handle if ctx is not None else None
+    k = _fn("zq_batch_workspace")(h, int(sz.size), sz.ctypes.data, ord(jobz), int(budget), ctypes.byref(nbytes),
+                                  run.ctypes.data)
+    if k < 0:
+        raise ZQError(k, "zq_batch_workspace")
+    return nbytes.value, k, [int(v) for v in run[: sz.size]]
+
+
 def zquatev(D, n2=None):
     """In-place drop-in for ts::zquatev on host memory (zquatev.cc:42).
 
     D: (ld2, >= n2) complex128, Fortran order; the left half of the leading
     n2 x n2 block holds [A; B].  Returns (info, eig[n2/2])."""
     D = _fortran_c128(D)
-    if n2 is None:
-        n2 = D.shape[1]
+    n2 = _host_n2(D, n2)
     ld2 = D.shape[0]
     eig = np.zeros(max(n2 // 2, 1), dtype=np.float64)
     info = _fn("zq_zquatev")(n2, D.ctypes.data, ld2, eig.ctypes.data)
@@ -155,6 +191,7 @@ class Context:
         _check(_fn("zq_create")(ctypes.byref(h), device), "zq_create")
         self._h = h
         self.device = device
+        self._stream = None     # the stream set with set_stream (None: the context's own)
         if panel_width:
             self.set_panel_width(panel_width)
 
@@ -183,15 +220,32 @@ class Context:
         """Run on the given HIP stream handle (0 = the null stream, which is
         torch's default stream)."""
         _check(_fn("zq_set_stream")(self._h, _vp(stream_ptr or 0)), "zq_set_stream")
+        self._stream = int(stream_ptr or 0)
 
     def use_own_stream(self):
         _check(_fn("zq_use_own_stream")(self._h), "zq_use_own_stream")
+        self._stream = None
 
     def set_panel_width(self, nb):
         _check(_fn("zq_set_panel_width")(self._h, int(nb)), "zq_set_panel_width")
 
     def synchronize(self):
         _check(_fn("zq_synchronize")(self._h), "zq_synchronize")
+
+    def last_info(self):
+        """Status of the last asynchronous solve on this context (waits for
+        its stream): 0, > 0 (no convergence), ERR_NONFINITE, ..."""
+        return int(_fn("zq_last_info")(self._h))
+
+    def set_iteration_caps(self, leaf=0, secular=0, bisect=0):
+        """Iteration caps of the tridiagonal solver (0: default 60 / 200 / 64);
+        reaching one is reported as non-convergence (info > 0)."""
+        _check(_fn("zq_set_iteration_caps")(self._h, int(leaf), int(secular), int(bisect)),
+               "zq_set_iteration_caps")
+
+    def set_workspace_limit(self, nbytes):
+        """Cap the device workspace this context may hold (0: no cap)."""
+        _check(_fn("zq_set_workspace_limit")(self._h, int(nbytes)), "zq_set_workspace_limit")
 
     def autotune(self, sizes):
         """Time the reduction with panel widths 16, 32, 64 and 128 for each
@@ -216,6 +270,20 @@ class Context:
         pool's contents and let the workspace be reallocated again."""
         _check(_fn("zq_graph_release")(self._h), "zq_graph_release")
 
+    def set_tile_shape(self, matvec_switch=-1, gemm64_tiles_per_cu=0):
+        """Override the matvec strip switch (trailing rows up to which narrow
+        strips are used, 0..2048) and the GEMM tile-rows threshold (64-row
+        tiles from this many 64 x 64 tiles per CU); -1 / 0 restore the tuned
+        (or default) values."""
+        _check(_fn("zq_set_tile_shape")(self._h, int(matvec_switch), int(gemm64_tiles_per_cu)),
+               "zq_set_tile_shape")
+
+    def tile_shape(self, n):
+        """(matvec strip switch, GEMM 64-row threshold) a solve of half-dimension n uses."""
+        a, b = _c_int(), _c_int()
+        _check(_fn("zq_tile_shape")(self._h, int(n), ctypes.byref(a), ctypes.byref(b)), "zq_tile_shape")
+        return a.value, b.value
+
     def panel_width(self, n):
         """Panel width a solve of half-dimension n on this context uses."""
         return int(_fn("zq_panel_width")(self._h, int(n)))
@@ -223,8 +291,7 @@ class Context:
     def zquatev(self, D, n2=None):
         """Host-memory solve on this context; same contract as :func:`zquatev`."""
         D = _fortran_c128(D)
-        if n2 is None:
-            n2 = D.shape[1]
+        n2 = _host_n2(D, n2)
         eig = np.zeros(max(n2 // 2, 1), dtype=np.float64)
         info = _fn("zq_zquatev_ctx")(self._h, n2, D.ctypes.data, D.shape[0], eig.ctypes.data)
         return info, eig[: n2 // 2]
@@ -237,6 +304,9 @@ class Context:
         """Host-memory batch: Ds is a list of column-major complex128 arrays
         (each solved in place, as :meth:`zquatev`).  Returns (rc, infos, eigs)."""
         Ds = [_fortran_c128(D) for D in Ds]
+        for D in Ds:
+            if D.shape[1] > D.shape[0]:
+                raise ValueError(f"D of shape {D.shape}: the order (columns) exceeds the leading dimension (rows)")
         cnt = len(Ds)
         n2 = (_c_int * max(cnt, 1))(*[D.shape[1] for D in Ds])
         ld2 = (_c_int * max(cnt, 1))(*[D.shape[0] for D in Ds])
@@ -330,6 +400,14 @@ class Server:
     def drain(self):
         _check(_fn("zq_server_drain")(self._h), "zq_server_drain")
 
+    def set_budget(self, gib):
+        """Device-memory budget of the lanes' workspaces in GiB (0: default,
+        80% of the memory free at creation)."""
+        _check(_fn("zq_server_set_budget")(self._h, float(gib)), "zq_server_set_budget")
+
+    def budget_gb(self):
+        return float(_fn("zq_server_budget_gb")(self._h))
+
     def clock_us(self):
         return _fn("zq_server_clock_us")(self._h)
 
@@ -379,13 +457,29 @@ def _torch():
 
 def solve_torch(ctx, D, jobz="V"):
     """Solve on a device tensor D (2n x 2n complex128, column-major, i.e. the
-    transpose of a contiguous row-major tensor is accepted as D.T).  Returns
-    (info, eig tensor); D is overwritten with the eigenvectors for jobz='V'."""
+    transpose of a contiguous row-major tensor is accepted as D.T), in order
+    on torch's current stream: the solve runs after the work already queued
+    there (the producers of D) and later work there sees its results.
+    Returns (rc, eig tensor); D is overwritten with the eigenvectors for
+    jobz='V'.  rc is the launch status (argument checks, allocation); the
+    solve's own status (convergence, non-finite input) is ctx.last_info(),
+    which waits for the stream."""
     torch = _torch()
     n2 = D.shape[0]
-    assert D.dtype == torch.complex128 and D.is_cuda
-    assert D.stride(0) == 1, "D must be column-major (use a transposed contiguous tensor)"
+    if D.dtype != torch.complex128 or not D.is_cuda or D.dim() != 2 or D.shape[1] != n2:
+        raise ValueError("D must be a square complex128 device tensor")
+    if D.stride(0) != 1:
+        raise ValueError("D must be column-major (use a transposed contiguous tensor)")
     ld2 = D.stride(1)
-    eig = torch.zeros(max(n2 // 2, 1), dtype=torch.float64, device=D.device)
-    rc = ctx.zquatev_device(n2, D.data_ptr(), ld2, eig.data_ptr(), jobz)
+    stream = torch.cuda.current_stream(D.device)
+    eig = torch.zeros(max(n2 // 2, 1), dtype=torch.float64, device=D.device)   # queued on `stream`
+    previous = ctx._stream
+    ctx.set_stream(stream.cuda_stream)
+    try:
+        rc = ctx.zquatev_device(n2, D.data_ptr(), ld2, eig.data_ptr(), jobz)
+    finally:
+        if previous is None:
+            ctx.use_own_stream()
+        else:
+            ctx.set_stream(previous)
     return rc, eig[: n2 // 2]

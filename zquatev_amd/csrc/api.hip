@@ -40,19 +40,36 @@ struct zq_context {
   hipStream_t own_stream = nullptr;
   int nb_override = 0;
   int tuned_nb[32] = {};      // autotuned panel width per bucket floor(log2 n) (0: none)
+  // tile shapes (per bucket, autotuned; -1 / 0: the default): the matvec
+  // strip switch (narrow strips up to this trailing size) and the GEMM tile
+  // rows threshold (64-row tiles from this many 64 x 64 tiles per CU)
+  int tuned_tsm[32] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                       -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int tuned_t64[32] = {};
+  int tsm_override = -1;
+  int t64_override = 0;
   bool ws_locked = false;     // captured graphs reference ws / binfo: no reallocation until released
   char* pin[2] = {nullptr, nullptr};   // pinned chunks of the host entry points' copies
   size_t pin_cap = 0;
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  size_t ws_limit = 0;        // most workspace bytes the context may hold (0: device memory decides)
+  // When set, a workspace being replaced by a larger one is kept (retired)
+  // instead of freed: hipFree waits for the whole device, which would stall
+  // the other contexts of a server.  The owner frees them while idle.
+  bool defer_free = false;
+  std::vector<void*> retired;
   void* dD = nullptr;       // staging copy of D for the host-memory entry points
   size_t dD_bytes = 0;
   double* deig = nullptr;
   size_t deig_bytes = 0;
   hipEvent_t ev[6] = {};
   bool have_times = false;
-  int* info_dev = nullptr;  // leaf-solver status of the last vectors solve
+  int* info_dev = nullptr;  // status word of the last single solve (device)
+  int last_kind = 0;        // 0 none, 1 single solve (info_dev), 2 batch (binfo, last_count entries)
+  int caps[3] = {0, 0, 0};  // iteration caps of the tridiagonal solver (0: default)
+  int last_count = 0;
   zq::ProfState prof;
   zq::Staging stage;
   int* binfo = nullptr;     // per-problem status of the last batch (device)
@@ -355,19 +372,48 @@ int panel_width(const zq_context* ctx, int n) {
   return nb;
 }
 
+constexpr int kTsmDefault = 768;   // matvec strip switch (trailing size)
+constexpr int kTile64Default = 16; // 64-row GEMM tiles from this many 64 x 64 tiles per CU
+
+int matvec_switch(const zq_context* ctx, int n) {
+  if (!ctx) return kTsmDefault;
+  if (ctx->tsm_override >= 0) return ctx->tsm_override;
+  const int t = ctx->tuned_tsm[size_bucket(n)];
+  return t >= 0 ? t : kTsmDefault;
+}
+
+int gemm64_min(const zq_context* ctx, int n) {
+  if (!ctx) return kTile64Default;
+  if (ctx->t64_override > 0) return ctx->t64_override;
+  const int t = ctx->tuned_t64[size_bucket(n)];
+  return t > 0 ? t : kTile64Default;
+}
+
 // ensure() for buffers that captured graphs may reference: while they are
 // held (or during a capture) a buffer can be used but not reallocated.
-int ensure_held(const zq_context* ctx, void*& ptr, size_t& have, size_t need) {
+int ensure_held(zq_context* ctx, void*& ptr, size_t& have, size_t need) {
   if ((ctx->ws_locked || ctx->stage.capture) && (!ptr || have < need)) return ZQ_ERR_GRAPH;
+  if (ctx->defer_free && ptr && have < need) {
+    ctx->retired.push_back(ptr);
+    ptr = nullptr;
+    have = 0;
+  }
   return ensure(ptr, have, need);
 }
 
 int make_plan(zq_context* ctx, int n, int nb, bool vectors, Plan* p) {
   const size_t need = layout_plan(n, nb, vectors, nullptr, p);
+  if (ctx->ws_limit && need > ctx->ws_limit) return ZQ_ERR_NOMEM;
   int rc = ensure_held(ctx, ctx->ws, ctx->ws_bytes, need);
   if (rc) return rc;
   layout_plan(n, nb, vectors, static_cast<char*>(ctx->ws), p);
   return ZQ_OK;
+}
+
+void apply_caps(const zq_context* ctx, zq::StedcWork& sw) {
+  if (ctx->caps[0] > 0) sw.cap_leaf = ctx->caps[0];
+  if (ctx->caps[1] > 0) sw.cap_secular = ctx->caps[1];
+  if (ctx->caps[2] > 0) sw.cap_bisect = ctx->caps[2];
 }
 
 int with_device(zq_context* ctx) {
@@ -386,14 +432,18 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   Plan p;
   int rc = make_plan(ctx, n, nb, vectors, &p);
   if (rc) return rc;
+  p.rw.tsm = matvec_switch(ctx, n);
+  zq::TileScope tiles(gemm64_min(ctx, n));
   ZQ_HIP_CHECK(mark(0));
-  if ((rc = zq::load_left_half(n, dD, ld2, p.M, p.ldm, s))) return rc;
+  ctx->last_kind = 0;
+  if ((rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
+  if ((rc = zq::load_left_half(n, dD, ld2, p.M, p.ldm, p.sw.info, s))) return rc;
   ZQ_HIP_CHECK(mark(1));
   if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
   if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
   ZQ_HIP_CHECK(mark(2));
-  if (vectors && (rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
-  ctx->info_dev = vectors ? p.sw.info : nullptr;
+  ctx->info_dev = p.sw.info;
+  ctx->last_kind = 1;
   if ((rc = zq::stage_begin(ctx->stage))) return rc;
   struct EndStage {
     zq_context* c;
@@ -402,6 +452,7 @@ int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool
   } end_stage{ctx, s};
   const int zero = 0;
   p.sw.stage = &ctx->stage;
+  apply_caps(ctx, p.sw);
   if ((rc = zq::stedc(1, &zero, &n, n, n, p.d, p.ea, deig, p.sw, vectors, s))) return rc;
   ZQ_HIP_CHECK(mark(3));
   if (vectors) {
@@ -471,7 +522,7 @@ size_t batch_layout(int nprob, const int* nh, int nb, bool vectors, char* base, 
 
 // One lockstep group on the context's workspace (already large enough).
 // P: the group's problems, sizes descending, P[z].off the forest offsets.
-int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vectors) {
+int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, int tsm, bool vectors) {
   hipStream_t s = ctx->stream;
   const int nprob = (int)P.size();
   std::vector<int> nh(nprob), zrow(nprob);
@@ -485,19 +536,22 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
   const int L = nh[0];
   BatchPlan bp;
   batch_layout(nprob, nh.data(), nb, vectors, static_cast<char*>(ctx->ws), &bp);
+  bp.slab.rw.tsm = tsm;
+  zq::TileScope tiles(gemm64_min(ctx, L));
   zq::Staging& st = ctx->stage;
   int rc;
   if ((rc = zq::stage_upload(st, nh.data(), nprob * sizeof(int), bp.nvec, s))) return rc;
   if ((rc = zq::stage_upload(st, zrow.data(), nprob * sizeof(int), bp.zrow, s))) return rc;
   if ((rc = zq::stage_upload(st, P.data(), nprob * sizeof(zq::BatchProb), bp.probs, s))) return rc;
   const zq::ZB zb{bp.nvec, bp.zs};
-  if ((rc = zq::load_left_batched(nprob, L, bp.probs, bp.slab.M, L, zb, s))) return rc;
+  if ((rc = zq::fill_zero(bp.sw.info, nprob * sizeof(int), s))) return rc;
+  if ((rc = zq::load_left_batched(nprob, L, bp.probs, bp.slab.M, L, zb, bp.sw.info, s))) return rc;
   if ((rc = zq::reduce_batched(nprob, nh.data(), zb, bp.slab.M, L, nb, bp.slab.d, bp.slab.e, bp.slab.rw, bp.rtasks,
                                bp.tstart, st, s)))
     return rc;
   if ((rc = zq::gather_tridiag_batched(nprob, L, bp.probs, bp.slab.d, bp.slab.e, zb, bp.dall, bp.eall, s))) return rc;
   bp.sw.stage = &st;
-  if (vectors && (rc = zq::fill_zero(bp.sw.info, nprob * sizeof(int), s))) return rc;
+  apply_caps(ctx, bp.sw);
   if ((rc = zq::stedc(nprob, zrow.data(), nh.data(), ntot, L, bp.dall, bp.eall, bp.wall, bp.sw, vectors, s)))
     return rc;
   if ((rc = zq::scatter_eig_batched(nprob, L, bp.probs, bp.wall, s))) return rc;
@@ -506,9 +560,8 @@ int solve_group(zq_context* ctx, std::vector<zq::BatchProb>& P, int nb, bool vec
                                         bp.slab.X, L, bp.slab.bw, bp.btasks, bp.sums, st, s)))
       return rc;
     if ((rc = zq::store_vectors_batched(nprob, L, bp.probs, bp.slab.X, L, zb, s))) return rc;
-    if ((rc = zq::scatter_info_batched(nprob, bp.probs, bp.sw.info, ctx->binfo, s))) return rc;
   }
-  return ZQ_OK;
+  return zq::scatter_info_batched(nprob, bp.probs, bp.sw.info, ctx->binfo, s);
 }
 
 // Device-memory batch: groups and solves (arguments already validated).
@@ -520,6 +573,7 @@ struct Groups {
   std::vector<std::vector<zq::BatchProb>> probs;   // per group, sizes descending
   std::vector<int> nb;                             // panel width of each group
   std::vector<int> wb;                             // back-transform block width of each group
+  std::vector<int> tsm;                            // matvec strip switch of each group
   size_t need = 0;                                 // workspace bytes of the largest group
 };
 
@@ -529,13 +583,15 @@ Groups make_groups(const zq_context* ctx, int count, const int* n2, double* cons
   for (int i = 0; i < count; ++i) {
     if (n2[i] == 0) continue;
     const int n = n2[i] / 2;
-    const int nb = panel_width(ctx, n);
+    const int nb = ctx ? panel_width(ctx, n) : zq_tuned_panel_width(n);
     const int wb = backtr_width(n);
+    const int tsm = matvec_switch(ctx, n);
     size_t g = 0;
-    while (g < G.nb.size() && !(G.nb[g] == nb && G.wb[g] == wb)) ++g;
+    while (g < G.nb.size() && !(G.nb[g] == nb && G.wb[g] == wb && G.tsm[g] == tsm)) ++g;
     if (g == G.nb.size()) {
       G.nb.push_back(nb);
       G.wb.push_back(wb);
+      G.tsm.push_back(tsm);
       G.probs.emplace_back();
     }
     G.probs[g].push_back(zq::BatchProb{D ? D[i] : nullptr, eig ? eig[i] : nullptr, ld2 ? ld2[i] : n2[i], n, 0, i});
@@ -552,6 +608,73 @@ Groups make_groups(const zq_context* ctx, int count, const int* n2, double* cons
   return G;
 }
 
+// Workspace of problems [a, b) of one group (sizes descending).
+size_t run_bytes(const std::vector<zq::BatchProb>& P, size_t a, size_t b, int nb, bool vectors) {
+  std::vector<int> nh(b - a);
+  for (size_t k = a; k < b; ++k) nh[k - a] = P[k].n;
+  BatchPlan bp;
+  return batch_layout((int)nh.size(), nh.data(), nb, vectors, nullptr, &bp);
+}
+
+// Splits every group into consecutive runs (sizes descending) whose workspace
+// fits `budget` bytes; each run is then solved as a lockstep batch of its own.
+// Every slab of a run is sized for the run's largest problem, so one large
+// problem no longer inflates the slabs of many smaller ones.  A problem that
+// does not fit even alone forms its own run (its allocation then fails).
+// The results do not change: each member computes exactly its solo solve.
+void split_groups(Groups& G, bool vectors, size_t budget) {
+  Groups out;
+  for (size_t g = 0; g < G.probs.size(); ++g) {
+    const std::vector<zq::BatchProb>& P = G.probs[g];
+    size_t a = 0;
+    while (a < P.size()) {
+      size_t lo = a + 1, hi = P.size();   // [a, lo) is always taken
+      if (run_bytes(P, a, hi, G.nb[g], vectors) <= budget) {
+        lo = hi;
+      } else {
+        while (hi - lo > 1) {   // the workspace grows with every member added
+          const size_t mid = (lo + hi) / 2;
+          if (run_bytes(P, a, mid, G.nb[g], vectors) <= budget) lo = mid;
+          else hi = mid;
+        }
+      }
+      out.probs.emplace_back(P.begin() + a, P.begin() + lo);
+      out.nb.push_back(G.nb[g]);
+      out.wb.push_back(G.wb[g]);
+      out.tsm.push_back(G.tsm[g]);
+      out.need = std::max(out.need, run_bytes(P, a, lo, G.nb[g], vectors));
+      a = lo;
+    }
+  }
+  G = std::move(out);
+}
+
+// Bytes of workspace a batch may use on ctx: the limit set on the context,
+// and what the device can give (the workspace held now, which is reused,
+// plus 90% of the free memory).  While captured graphs hold the workspace,
+// exactly what is held.
+size_t workspace_budget(zq_context* ctx) {
+  if (ctx->ws_locked || ctx->stage.capture) return ctx->ws_bytes;
+  size_t budget = SIZE_MAX;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = ctx->ws_bytes + (size_t)(0.9 * (double)fr);
+  else (void)hipGetLastError();
+  if (ctx->ws_limit) budget = std::min(budget, ctx->ws_limit);
+  return budget;
+}
+
+// Aggregate status of a batch from the per-problem codes: ZQ_ERR_NONFINITE
+// if some input held a NaN or an infinity, else 1 if some problem did not
+// converge, else 0.
+int batch_status(const int* info, int count) {
+  int agg = 0;
+  for (int i = 0; i < count; ++i) {
+    if (info[i] == ZQ_ERR_NONFINITE) return ZQ_ERR_NONFINITE;
+    if (info[i] > 0) agg = 1;
+  }
+  return agg;
+}
+
 int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
                   bool vectors, int* info, int* info_async) {
   Groups G = make_groups(ctx, count, n2, D, ld2, eig, vectors);
@@ -562,18 +685,28 @@ int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, c
       for (int i = 0; i < count; ++i) info_async[i] = 0;
     return ZQ_OK;
   }
+  if (G.need > ctx->ws_bytes || (ctx->ws_limit && G.need > ctx->ws_limit)) {
+    const size_t budget = workspace_budget(ctx);
+    if (G.need > budget) split_groups(G, vectors, budget);
+  }
   const size_t need = G.need;
+  if (ctx->ws_limit && need > ctx->ws_limit) return ZQ_ERR_NOMEM;   // a single problem over the limit
   int rc = ensure_held(ctx, ctx->ws, ctx->ws_bytes, need);
   if (rc) return rc;
   if ((rc = ensure_held(ctx, reinterpret_cast<void*&>(ctx->binfo), ctx->binfo_bytes,
                         (size_t)count * sizeof(int) + 256)))
     return rc;
   ctx->have_times = false;
+  ctx->prof.used = 0;
+  zq::g_prof = ctx->stage.capture ? nullptr : &ctx->prof;   // per-kernel-class events when instrumented
+  struct ResetProf { ~ResetProf() { zq::g_prof = nullptr; } } reset_prof;
   ctx->info_dev = nullptr;
+  ctx->last_kind = 2;
+  ctx->last_count = count;
   hipStream_t s = ctx->stream;
   if ((rc = zq::fill_zero(ctx->binfo, (size_t)count * sizeof(int), s))) return rc;
   if ((rc = zq::stage_begin(ctx->stage))) return rc;
-  for (size_t g = 0; g < G.probs.size() && !rc; ++g) rc = solve_group(ctx, G.probs[g], G.nb[g], vectors);
+  for (size_t g = 0; g < G.probs.size() && !rc; ++g) rc = solve_group(ctx, G.probs[g], G.nb[g], G.tsm[g], vectors);
   const int rc2 = zq::stage_end(ctx->stage, s);
   if (rc) return rc;
   if (rc2) return rc2;
@@ -582,9 +715,8 @@ int solve_batched(zq_context* ctx, int count, const int* n2, double* const* D, c
   if (!info) return ZQ_OK;
   ZQ_HIP_CHECK(hipMemcpyAsync(info, ctx->binfo, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, s));
   ZQ_HIP_CHECK(hipStreamSynchronize(s));
-  for (int i = 0; i < count; ++i)
-    if (info[i] != 0) return 1;
-  return ZQ_OK;
+  for (int i = 0; i < count; ++i) info[i] = info_code(info[i]);
+  return batch_status(info, count);
 }
 
 int check_batch(const zq_context* c, int count, const int* n2, double* const* D, const int* ld2, double* const* eig,
@@ -776,6 +908,8 @@ const char* zq_error_string(int code) {
     case ZQ_ERR_HIP: return "HIP runtime error";
     case ZQ_ERR_NOMEM: return "device memory exhausted";
     case ZQ_ERR_ARG: return "unsupported internal configuration";
+    case ZQ_ERR_NONFINITE: return "the input holds a NaN or an infinity (or overflowed the reduction)";
+    case ZQ_ERR_BUDGET: return "the request needs more device memory than the server's budget";
     case ZQ_ERR_GRAPH:
       return "graph capture: this call cannot be captured, its workspace is not allocated yet (run it once "
              "outside capture), or the upload pool / workspace is held by captured graphs (zq_graph_reserve, "
@@ -862,6 +996,7 @@ int zq_destroy(zq_context* c) {
     if (c->pin_ev[k]) (void)hipEventDestroy(c->pin_ev[k]);
   }
   if (c->ws) (void)hipFree(c->ws);
+  for (void* p : c->retired) (void)hipFree(p);
   if (c->dD) (void)hipFree(c->dD);
   if (c->deig) (void)hipFree(c->deig);
   if (c->binfo) (void)hipFree(c->binfo);
@@ -917,10 +1052,14 @@ int zq_panel_width(zq_context* c, int n) {
   return panel_width(c, n);
 }
 
-// Times the reduction (the one phase the panel width changes) for each
-// candidate width on a synthetic U(-1,1) input generated on the device: one
-// warm-up and two timed runs each, the faster kept.  Candidates wider than
-// the matrix are skipped (they would repeat the previous one's schedule).
+// Times the reduction (the phase the panel width and the tile shapes change)
+// on a synthetic U(-1,1) input generated on the device: one warm-up and two
+// timed runs per candidate, the faster kept.  First the panel width (widths
+// wider than the matrix are skipped: they would repeat the previous one's
+// schedule), then with it the matvec strip switch (switches at or above the
+// largest trailing size all give the same schedule: the first of them
+// stands for the rest), then the GEMM tile-rows threshold.  Each pick is
+// recorded for the size's bucket unless an override is set.
 int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double* ms) try {
   if (!c) return ZQ_ERR_ARG;
   if (count < 0) return -3;
@@ -936,6 +1075,8 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
     if (cap.active) return ZQ_ERR_GRAPH;   // it waits on its timing events
   }
   static const int cand[4] = {16, 32, 64, 128};
+  static const int cand_tsm[4] = {512, 768, 1024, 1536};
+  static const int cand_t64[3] = {4, 16, 64};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreateWithFlags(&e0, zq::kTimingEvent) != hipSuccess || hipEventCreateWithFlags(&e1, zq::kTimingEvent) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
@@ -946,25 +1087,37 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
     ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
   } ev{e0, e1};
   hipStream_t s = c->stream;
+  // best of two timed reductions after a warm-up (ms), or < 0 with rc set
+  auto time_reduce = [&](int n, int nb, int tsm, int t64, double* out) -> int {
+    Plan p;
+    int r = make_plan(c, n, nb, false, &p);
+    if (r) return r;
+    p.rw.tsm = tsm;
+    zq::TileScope tiles(t64);
+    double tmin = -1.0;
+    for (int rep = 0; rep < 3; ++rep) {
+      if ((r = zq::gen_lower_uniform(n, 1001, p.M, p.ldm, s))) return r;
+      ZQ_HIP_CHECK(hipEventRecord(e0, s));
+      if ((r = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return r;
+      ZQ_HIP_CHECK(hipEventRecord(e1, s));
+      ZQ_HIP_CHECK(hipEventSynchronize(e1));
+      float f = 0.f;
+      ZQ_HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
+      if (rep > 0 && (tmin < 0.0 || f < tmin)) tmin = f;
+    }
+    *out = tmin;
+    return ZQ_OK;
+  };
   for (int k = 0; k < count; ++k) {
     const int n = sizes[k];
+    const int bucket = size_bucket(n);
+    int tsm = matvec_switch(c, n), t64 = gemm64_min(c, n);
     int best = 0;
     double best_ms = 0.0;
     for (int t = 0; t < 4; ++t) {
       double tmin = -1.0;
       if (t == 0 || cand[t - 1] < n - 1) {
-        Plan p;
-        if ((rc = make_plan(c, n, cand[t], false, &p))) return rc;
-        for (int rep = 0; rep < 3; ++rep) {
-          if ((rc = zq::gen_lower_uniform(n, 1001, p.M, p.ldm, s))) return rc;
-          ZQ_HIP_CHECK(hipEventRecord(e0, s));
-          if ((rc = zq::reduce(p.M, n, p.ldm, cand[t], p.d, p.e, p.rw, s))) return rc;
-          ZQ_HIP_CHECK(hipEventRecord(e1, s));
-          ZQ_HIP_CHECK(hipEventSynchronize(e1));
-          float f = 0.f;
-          ZQ_HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
-          if (rep > 0 && (tmin < 0.0 || f < tmin)) tmin = f;
-        }
+        if ((rc = time_reduce(n, cand[t], tsm, t64, &tmin))) return rc;
         if (best == 0 || tmin < best_ms) {
           best = cand[t];
           best_ms = tmin;
@@ -972,12 +1125,57 @@ int zq_autotune(zq_context* c, const int* sizes, int count, int* chosen, double*
       }
       if (ms) ms[4 * k + t] = tmin;
     }
-    c->tuned_nb[size_bucket(n)] = best;
+    c->tuned_nb[bucket] = best;
     if (chosen) chosen[k] = best;
+    const int nb = panel_width(c, n);   // the pick, or the caller's override
+    if (c->tsm_override < 0 && n - 1 > cand_tsm[0]) {
+      int pick = tsm;
+      double pick_ms = -1.0;
+      for (int t = 0; t < 4; ++t) {
+        if (t > 0 && cand_tsm[t - 1] >= n - 1) break;   // same schedule as the previous candidate
+        double tm = -1.0;
+        if ((rc = time_reduce(n, nb, cand_tsm[t], t64, &tm))) return rc;
+        if (pick_ms < 0.0 || tm < pick_ms) {
+          pick = cand_tsm[t];
+          pick_ms = tm;
+        }
+      }
+      c->tuned_tsm[bucket] = pick;
+      tsm = pick;
+    }
+    if (c->t64_override <= 0) {
+      int pick = t64;
+      double pick_ms = -1.0;
+      for (int t = 0; t < 3; ++t) {
+        double tm = -1.0;
+        if ((rc = time_reduce(n, nb, tsm, cand_t64[t], &tm))) return rc;
+        if (pick_ms < 0.0 || tm < pick_ms) {
+          pick = cand_t64[t];
+          pick_ms = tm;
+        }
+      }
+      c->tuned_t64[bucket] = pick;
+    }
   }
   return ZQ_OK;
 } catch (...) {
   return exception_status();
+}
+
+int zq_set_tile_shape(zq_context* c, int matvec_switch_rows, int gemm64_tiles_per_cu) {
+  if (!c) return ZQ_ERR_ARG;
+  if (matvec_switch_rows < -1 || matvec_switch_rows > 2048) return -2;
+  if (gemm64_tiles_per_cu < 0) return -3;
+  c->tsm_override = matvec_switch_rows;
+  c->t64_override = gemm64_tiles_per_cu;
+  return ZQ_OK;
+}
+
+int zq_tile_shape(zq_context* c, int n, int* matvec_switch_rows, int* gemm64_tiles_per_cu) {
+  if (!c || n < 1) return ZQ_ERR_ARG;
+  if (matvec_switch_rows) *matvec_switch_rows = matvec_switch(c, n);
+  if (gemm64_tiles_per_cu) *gemm64_tiles_per_cu = gemm64_min(c, n);
+  return ZQ_OK;
 }
 
 int zq_zquatev_device(zq_context* c, int n2, double* dD, int nld2, double* deig, int jobz) try {
@@ -1015,7 +1213,7 @@ int zq_zquatev_ctx(zq_context* c, int n2, double* D, int nld2, double* eig) try 
   int info = 0;
   if (c->info_dev) ZQ_HIP_CHECK(hipMemcpyAsync(&info, c->info_dev, sizeof(int), hipMemcpyDeviceToHost, s));
   ZQ_HIP_CHECK(hipStreamSynchronize(s));
-  return info;
+  return info_code(info);
 } catch (...) {
   return exception_status();
 }
@@ -1103,8 +1301,13 @@ int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double*
   const int nb = panel_width(c, n);
   Plan p;
   if ((rc = make_plan(c, n, nb, false, &p))) return rc;
+  p.rw.tsm = matvec_switch(c, n);
+  zq::TileScope tiles(gemm64_min(c, n));
   hipStream_t s = c->stream;
-  if ((rc = zq::load_left_half(n, dD, nld2, p.M, p.ldm, s))) return rc;
+  if ((rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
+  if ((rc = zq::load_left_half(n, dD, nld2, p.M, p.ldm, p.sw.info, s))) return rc;
+  c->info_dev = p.sw.info;
+  c->last_kind = 1;
   if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
   if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
   ZQ_HIP_CHECK(hipMemcpyAsync(dd, p.d, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1126,9 +1329,12 @@ int zq_tridiag_eig(zq_context* c, int n, const double* dd, const double* de, dou
   const bool vec = dZ != nullptr;
   if ((rc = make_plan(c, n, 32, vec, &p))) return rc;
   hipStream_t s = c->stream;
-  if (vec && (rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
+  if ((rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;
+  c->info_dev = p.sw.info;
+  c->last_kind = 1;
   const int zero = 0;
   p.sw.stage = &c->stage;
+  apply_caps(c, p.sw);
   if ((rc = zq::stage_begin(c->stage))) return rc;
   if ((rc = zq::stedc(1, &zero, &n, n, n, dd, de, dw, p.sw, vec, s))) return rc;
   if ((rc = zq::stage_end(c->stage, s))) return rc;
@@ -1193,6 +1399,62 @@ int zq_synchronize(zq_context* c) {
   return hipStreamSynchronize(c->stream) == hipSuccess ? ZQ_OK : ZQ_ERR_HIP;
 }
 
+int zq_last_info(zq_context* c) try {
+  if (!c) return ZQ_ERR_ARG;
+  if (with_device(c)) return ZQ_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
+  if (c->last_kind == 1 && c->info_dev) {
+    int w = 0;
+    ZQ_HIP_CHECK(hipMemcpy(&w, c->info_dev, sizeof(int), hipMemcpyDeviceToHost));
+    return info_code(w);
+  }
+  if (c->last_kind == 2 && c->binfo && c->last_count > 0) {
+    std::vector<int> w(c->last_count);
+    ZQ_HIP_CHECK(hipMemcpy(w.data(), c->binfo, w.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int& v : w) v = info_code(v);
+    return batch_status(w.data(), c->last_count);
+  }
+  return ZQ_OK;
+} catch (...) {
+  return exception_status();
+}
+
+int zq_set_iteration_caps(zq_context* c, int leaf, int secular, int bisect) {
+  if (!c) return ZQ_ERR_ARG;
+  if (leaf < 0 || secular < 0 || bisect < 0) return -2;
+  c->caps[0] = leaf;
+  c->caps[1] = secular;
+  c->caps[2] = bisect;
+  return ZQ_OK;
+}
+
+int zq_set_workspace_limit(zq_context* c, size_t bytes) {
+  if (!c) return ZQ_ERR_ARG;
+  c->ws_limit = bytes;
+  return ZQ_OK;
+}
+
+int zq_batch_workspace(zq_context* c, int count, const int* n2, int jobz, size_t budget, size_t* bytes,
+                       int* run) try {
+  if (count < 0) return -2;
+  if (count > 0 && !n2) return -3;
+  if (jobz != 'V' && jobz != 'v' && jobz != 'N' && jobz != 'n') return -4;
+  for (int i = 0; i < count; ++i)
+    if (n2[i] < 0 || (n2[i] & 1)) return -3;
+  const bool vectors = jobz == 'V' || jobz == 'v';
+  Groups G = make_groups(c, count, n2, nullptr, nullptr, nullptr, vectors);
+  if (budget > 0 && G.need > budget) split_groups(G, vectors, budget);
+  if (bytes) *bytes = G.need;
+  if (run) {
+    for (int i = 0; i < count; ++i) run[i] = -1;
+    for (size_t g = 0; g < G.probs.size(); ++g)
+      for (const zq::BatchProb& b : G.probs[g]) run[b.idx] = (int)g;
+  }
+  return (int)G.probs.size();
+} catch (...) {
+  return exception_status();
+}
+
 }  // extern "C"
 
 namespace ts {
@@ -1217,4 +1479,49 @@ size_t batched_workspace(const zq_context* c, int count, const int* n2, int jobz
 }
 
 hipStream_t context_stream(const zq_context* c) { return c->stream; }
+
+int context_reserve(zq_context* c, int max_count, size_t stage_bytes) {
+  int rc;
+  if ((rc = with_device(c))) return rc;
+  if ((rc = ensure(reinterpret_cast<void*&>(c->binfo), c->binfo_bytes, (size_t)max_count * sizeof(int) + 256)))
+    return rc;
+  for (int k = 0; k < 2; ++k) {
+    if (c->stage.cap[k] >= stage_bytes) continue;
+    if (c->stage.host[k]) {
+      if (hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
+      (void)hipHostFree(c->stage.host[k]);
+      c->stage.host[k] = nullptr;
+      c->stage.cap[k] = 0;
+    }
+    void* h = nullptr;
+    if (hipHostMalloc(&h, stage_bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return ZQ_ERR_NOMEM;
+    }
+    c->stage.host[k] = static_cast<char*>(h);
+    c->stage.cap[k] = stage_bytes;
+  }
+  return ZQ_OK;
+}
+
+void context_set_defer_free(zq_context* c, bool on) { c->defer_free = on; }
+
+int context_free_retired(zq_context* c) {
+  if (c->retired.empty()) return ZQ_OK;
+  if (with_device(c)) return ZQ_ERR_HIP;
+  int rc = ZQ_OK;
+  for (void* p : c->retired)
+    if (hipFree(p) != hipSuccess) rc = ZQ_ERR_HIP;
+  c->retired.clear();
+  return rc;
+}
+
+int context_release_workspace(zq_context* c) {
+  if (c->ws_locked || !c->ws) return ZQ_OK;
+  if (with_device(c)) return ZQ_ERR_HIP;
+  const hipError_t e = hipFree(c->ws);
+  c->ws = nullptr;
+  c->ws_bytes = 0;
+  return e == hipSuccess ? ZQ_OK : ZQ_ERR_HIP;
+}
 }  // namespace zq

@@ -273,7 +273,9 @@ static int device_cus() {
   return cached[dev];
 }
 
-static int tile_rows(long tiles64) { return tiles64 >= 16L * device_cus() ? 64 : 32; }
+thread_local int g_tile64_per_cu = 16;
+
+static int tile_rows(long tiles64) { return tiles64 >= (long)g_tile64_per_cu * device_cus() ? 64 : 32; }
 
 template <int TBM>
 static void launch_gemm2_t(bool act, bool bct, hipStream_t s, const QGemmTask& t, int nz, int kchunk,

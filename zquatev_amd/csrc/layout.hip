@@ -10,15 +10,30 @@
 
 namespace zq {
 
+// An input entry that is a NaN or an infinity becomes 0 here and marks the
+// problem's status word, so that the solve runs on finite data (the divide &
+// conquer ranks values by comparisons, which a NaN would make inconsistent)
+// and the entry point reports ZQ_ERR_NONFINITE.
+__device__ __forceinline__ quat finite_or_flag(quat q, int* info) {
+  if (!(isfinite(q.ar) && isfinite(q.ai) && isfinite(q.br) && isfinite(q.bi))) {
+    q.ar = isfinite(q.ar) ? q.ar : 0.0;
+    q.ai = isfinite(q.ai) ? q.ai : 0.0;
+    q.br = isfinite(q.br) ? q.br : 0.0;
+    q.bi = isfinite(q.bi) ? q.bi : 0.0;
+    atomicOr(info, ZQI_INFO_NONFINITE);
+  }
+  return q;
+}
+
 __global__ void __launch_bounds__(256) k_load_left(int n, const double* __restrict__ D, int ld2, quat* __restrict__ M,
-                                                   int ld) {
+                                                   int ld, int* __restrict__ info) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= n || i < j) return;
   const double2* col = reinterpret_cast<const double2*>(D) + (size_t)j * ld2;
   const double2 a = col[i];
   const double2 b = col[n + i];
-  qstore(M + i + (size_t)j * ld, quat{a.x, a.y, b.x, b.y});
+  qstore(M + i + (size_t)j * ld, finite_or_flag(quat{a.x, a.y, b.x, b.y}, info));
 }
 
 __global__ void __launch_bounds__(256) k_store_vec(int n, const quat* __restrict__ X, int ldx, double* __restrict__ D,
@@ -42,7 +57,7 @@ __global__ void k_offdiag_abs(int n, const quat* __restrict__ e, double* __restr
 
 // ---- batches: problem blockIdx.z, its table entry P[z], its buffers by zb ----
 __global__ void __launch_bounds__(256) k_load_left_b(const BatchProb* __restrict__ P, quat* __restrict__ M, int ld,
-                                                     ZB zb) {
+                                                     ZB zb, int* __restrict__ info) {
   const int n = P[blockIdx.z].n;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int j = blockIdx.y;
@@ -51,7 +66,7 @@ __global__ void __launch_bounds__(256) k_load_left_b(const BatchProb* __restrict
   const double2* col = reinterpret_cast<const double2*>(P[blockIdx.z].D) + (size_t)j * ld2;
   const double2 a = d2load_g(col + i);
   const double2 b = d2load_g(col + n + i);
-  qstore(zoff(M, zb) + i + (size_t)j * ld, quat{a.x, a.y, b.x, b.y});
+  qstore(zoff(M, zb) + i + (size_t)j * ld, finite_or_flag(quat{a.x, a.y, b.x, b.y}, info + blockIdx.z));
 }
 
 __global__ void __launch_bounds__(256) k_store_vec_b(const BatchProb* __restrict__ P, const quat* __restrict__ X,
@@ -143,9 +158,9 @@ int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* in
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
-int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s) {
+int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, int* info, hipStream_t s) {
   if (nprob <= 0 || L <= 0) return ZQI_OK;
-  hipLaunchKernelGGL(k_load_left_b, dim3((L + 255) / 256, L, nprob), dim3(256), 0, s, P, M, ld, zb);
+  hipLaunchKernelGGL(k_load_left_b, dim3((L + 255) / 256, L, nprob), dim3(256), 0, s, P, M, ld, zb, info);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
@@ -168,8 +183,8 @@ int scatter_eig_batched(int nprob, int L, const BatchProb* P, const double* wall
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
-int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s) {
-  hipLaunchKernelGGL(k_load_left, dim3((n + 255) / 256, n), dim3(256), 0, s, n, D, ld2, M, ld);
+int load_left_half(int n, const double* D, int ld2, quat* M, int ld, int* info, hipStream_t s) {
+  hipLaunchKernelGGL(k_load_left, dim3((n + 255) / 256, n), dim3(256), 0, s, n, D, ld2, M, ld, info);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

@@ -28,13 +28,15 @@ namespace zq {
 constexpr int RB = 256;      // threads per block of the row kernels
 constexpr int RQ = 16;       // rows per block of the row kernels (16 threads per row split the K sums)
 constexpr int SG = 64;       // matvec rows per block (one per lane)
-// Matvec tile shapes: SG rows x SW columns.  Trailing sizes up to TSM use
-// narrow strips (SW = 16, 256 threads: more, smaller blocks, which keeps the
-// small matvecs of batches and of the end of a solve spread over the CUs);
-// larger ones wide strips (SW = 32, 512 threads: fewer row partials for the
-// column kernel to gather).  spg = SG / SW strips per row group.
-constexpr int TSM = 768;
-__host__ __device__ constexpr int matvec_spg(int m) { return m <= TSM ? 4 : 2; }
+// Matvec tile shapes: SG rows x SW columns.  Trailing sizes up to the switch
+// tsm use narrow strips (SW = 16, 256 threads: more, smaller blocks, which
+// keeps the small matvecs of batches and of the end of a solve spread over
+// the CUs); larger ones wide strips (SW = 32, 512 threads: fewer row
+// partials for the column kernel to gather).  spg = SG / SW strips per row
+// group.  The switch is a per-context tuning parameter (0 .. TSM_MAX; the
+// autotuner times candidates), 768 by default.
+constexpr int TSM_MAX = 2048;
+__host__ __device__ constexpr int matvec_spg(int m, int tsm) { return m <= tsm ? 4 : 2; }
 // matvec tiles of a trailing size with G row groups: group g holds spg (g + 1) strips
 __host__ __device__ constexpr int ntiles(int G, int spg) { return spg * G * (G + 1) / 2; }
 constexpr int NBMAX = 128;   // largest supported panel width
@@ -319,7 +321,8 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
                                              quat* __restrict__ W, int ldw, const quat* __restrict__ Prow,
                                              const quat* __restrict__ Pcol, int ldp, const quat* __restrict__ TS,
                                              int nb, const double* __restrict__ scal, const double* __restrict__ QF,
-                                             double* __restrict__ part, double* __restrict__ TSP, int ldtsp, ZB zb) {
+                                             double* __restrict__ part, double* __restrict__ TSP, int ldtsp, int tsm,
+                                             ZB zb) {
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     if (c + 1 + (int)blockIdx.x * RQ >= n) return;
@@ -370,7 +373,7 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
   //   x^* M x: QF[tid + RB t], t over the G (G + 1) tile partials;
   //   (M x)_{s0}: Prow[0][s0] and Pcol[g][s0] as slots tid + RB t of G + 1.
   const int li = ic - s0;
-  const int spg = matvec_spg(n - s0);
+  const int spg = matvec_spg(n - s0, tsm);
   const int nJ = (spg == 4 ? li >> 4 : li >> 5) + 1, g0 = li / SG;
   const int cy = cdiv_d(nJ + G - g0 - q, 16);
   const int nq = ntiles(G, spg);
@@ -649,10 +652,11 @@ __global__ void k_last_diag(const quat* __restrict__ M, int ld, int n, double* _
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-int matvec_row_slots(int nmax) { return std::max(cdiv(nmax, 32), cdiv(std::min(nmax, TSM), 16)); }
+// (sized for any switch up to TSM_MAX)
+int matvec_row_slots(int nmax) { return std::max(cdiv(nmax, 32), cdiv(std::min(nmax, TSM_MAX), 16)); }
 int matvec_col_slots(int nmax) { return (nmax + SG - 1) / SG; }
 int matvec_tile_slots(int nmax) {
-  return std::max(ntiles(cdiv(nmax, SG) + 1, 2), ntiles(cdiv(std::min(nmax, TSM), SG) + 1, 4));
+  return std::max(ntiles(cdiv(nmax, SG) + 1, 2), ntiles(cdiv(std::min(nmax, TSM_MAX), SG) + 1, 4));
 }
 
 // panel columns per thread (16 threads per row) as a compile-time bound
@@ -686,7 +690,7 @@ static void launch_wcol(int kpt, int sz, dim3 grid, hipStream_t s, Args... args)
 // trailing update.
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s) {
   if (n <= 0) return ZQI_OK;
-  if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
+  if (nb < 1 || nb > NBMAX || w.tsm < 0 || w.tsm > TSM_MAX) return ZQI_ERR_ARG;
   if (n == 1) {
     hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, s, M, ld, n, d, ZB{nullptr, 0});
     return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
@@ -703,7 +707,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
         hipLaunchKernelGGL(k_col_norm, dim3(cdiv(n - c, 256)), dim3(256), 0, s, M, ld, n, c, w.part, ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
-      const int spg = matvec_spg(sz);
+      const int spg = matvec_spg(sz, w.tsm);
       const int nmain = ntiles(G, spg) + (j > 0 ? (8 * j + 63) / 64 : 1);
       {
         // algorithmic bytes: one read of the trailing lower triangle
@@ -716,7 +720,7 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
                    w.scal, d, e, w.Prow, w.Pcol, n, w.TS, nb, w.TSP, w.ldtsp, w.QF, ZB{nullptr, 0});
       }
       launch_wcol(kpt, sz, dim3(cdiv(sz, RQ)), s, M, ld, n, p, c, j, w.W, ldw, w.Prow, w.Pcol, n, w.TS, nb, w.scal,
-                  w.QF, w.part, w.TSP, w.ldtsp, ZB{nullptr, 0});
+                  w.QF, w.part, w.TSP, w.ldtsp, w.tsm, ZB{nullptr, 0});
     }
     // trailing update of rows/cols [p+wdt, n): M -= [V W] [W V]^*  (lower triangle)
     const int r0 = p + wdt;
@@ -750,7 +754,7 @@ size_t reduce_batched_tasks(int nprob, const int* nh, int nb) {
 int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, double* d, quat* e, const ReduceWork& w,
                    QGemmTask* tasks, int* tstart, Staging& st, hipStream_t s) {
   if (nprob <= 0) return ZQI_OK;
-  if (nb < 1 || nb > NBMAX) return ZQI_ERR_ARG;
+  if (nb < 1 || nb > NBMAX || w.tsm < 0 || w.tsm > TSM_MAX) return ZQI_ERR_ARG;
   const int L = nh[0];
   for (int z = 1; z < nprob; ++z)
     if (nh[z] > nh[z - 1] || nh[z] < 1) return ZQI_ERR_ARG;
@@ -767,6 +771,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
   // problem z in panel P is what reduce() launches for z after that panel
   std::vector<QGemmTask> tt;
   std::vector<int> toff(npan + 1, 0), tmaxm(npan, 0);
+  std::vector<double> tflops(npan, 0.0);   // algorithmic flops of each panel's trailing updates
   for (int P = 0; P < npan; ++P) {
     const int p = P * nb;
     toff[P] = (int)tt.size();
@@ -782,6 +787,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
       QOperand B{Wz + r0, Mz + r0 + (size_t)p * ld, ldw, ld, wdt, true, INT_MIN / 2};
       tt.push_back(QGemmTask{mt, mt, 2 * wdt, 0, 2 * wdt, ld, 1, -1.0, 1.0, A, B, Mz + r0 + (size_t)r0 * ld});
       tmaxm[P] = std::max(tmaxm[P], mt);
+      tflops[P] += 32.0 * 0.5 * (double)mt * (mt + 1) * 2.0 * wdt;
     }
   }
   toff[npan] = (int)tt.size();
@@ -804,24 +810,33 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     // each problem's matvec takes the tile shape of its own trailing size:
     // problems [0, k1) wide strips, [k1, act) narrow ones (sizes descending)
     int k1 = 0;
-    while (k1 < act && matvec_spg(nh[k1] - c - 1) == 2) ++k1;
-    if (k1 > 0)
-      hipLaunchKernelGGL((k_symv<512, 32>), dim3(ntiles(cdiv(sz, SG), 2) + NF, 1, k1), dim3(512), 0, s, M, ld, L,
-                         p, c, j, w.W, ldw, w.part, nblk, w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp,
-                         w.QF, zb);
+    while (k1 < act && matvec_spg(nh[k1] - c - 1, w.tsm) == 2) ++k1;
+    // algorithmic bytes of a launch: one read of each member's trailing lower triangle
+    auto tri_bytes = [&](int z0, int z1) {
+      double b = 0.0;
+      for (int z = z0; z < z1; ++z) b += 32.0 * 0.5 * (double)(nh[z] - c - 1) * (nh[z] - c);
+      return b;
+    };
+    if (k1 > 0) {
+      ProfScope ps(PROF_SYMV, s, tri_bytes(0, k1));
+      launch_k(k_symv<512, 32>, dim3(ntiles(cdiv(sz, SG), 2) + NF, 1, k1), dim3(512), 0, s, M, ld, L, p, c, j, w.W,
+               ldw, w.part, nblk, w.scal, d, e, w.Prow, w.Pcol, ld, w.TS, nb, w.TSP, w.ldtsp, w.QF, zb);
+    }
     if (k1 < act) {
       const size_t sh = (size_t)k1 * zb.zs;
       const int szn = nh[k1] - c - 1;
-      hipLaunchKernelGGL((k_symv<256, 16>), dim3(ntiles(cdiv(szn, SG), 4) + NF, 1, act - k1), dim3(256), 0, s,
-                         zshift(M, sh), ld, L, p, c, j, zshift(w.W, sh), ldw, zshift(w.part, sh), nblk,
-                         zshift(w.scal, sh), zshift(d, sh), zshift(e, sh), zshift(w.Prow, sh), zshift(w.Pcol, sh), ld,
-                         zshift(w.TS, sh), nb, zshift(w.TSP, sh), w.ldtsp, zshift(w.QF, sh), ZB{zb.nvec + k1, zb.zs});
+      ProfScope ps(PROF_SYMV, s, tri_bytes(k1, act));
+      launch_k(k_symv<256, 16>, dim3(ntiles(cdiv(szn, SG), 4) + NF, 1, act - k1), dim3(256), 0, s, zshift(M, sh), ld,
+               L, p, c, j, zshift(w.W, sh), ldw, zshift(w.part, sh), nblk, zshift(w.scal, sh), zshift(d, sh),
+               zshift(e, sh), zshift(w.Prow, sh), zshift(w.Pcol, sh), ld, zshift(w.TS, sh), nb, zshift(w.TSP, sh),
+               w.ldtsp, zshift(w.QF, sh), ZB{zb.nvec + k1, zb.zs});
     }
     launch_wcol(kpt, sz, dim3(cdiv(sz, RQ), 1, act), s, M, ld, L, p, c, j, w.W, ldw, w.Prow, w.Pcol, ld, w.TS, nb,
-                w.scal, w.QF, w.part, w.TSP, w.ldtsp, zb);
+                w.scal, w.QF, w.part, w.TSP, w.ldtsp, w.tsm, zb);
     if (j == nb - 1 || c == cend - 1) {
       const int P = c / nb;
       const int cnt = toff[P + 1] - toff[P];
+      ProfScope ps(PROF_TRAIL, s, tflops[P]);
       if (cnt > 0 && (rc = qgemm_tasks(tasks + toff[P], cnt, tmaxm[P], tmaxm[P], false, true, true, s))) return rc;
     }
   }

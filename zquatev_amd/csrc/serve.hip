@@ -12,7 +12,18 @@
 // further requests of that jobz and class join in arrival order, up to
 // max_batch problems and max_bytes of workspace.  Batches of the large
 // classes (half-dimension above 640) occupy at most all lanes but one, so
-// small requests always find a lane soon.  Each lane has its own
+// small requests always find a lane soon.
+//
+// Device memory: the lanes' workspaces together stay within a budget (by
+// default 80% of the device memory free when the server is created).  A lane
+// whose batch needs a larger workspace allocates it at once and keeps the old
+// one (retired) until no lane has work in flight: hipFree waits for the whole
+// device, so freeing while other lanes run would stall them.  Retired bytes
+// count against the budget until they are freed.  A batch that would exceed
+// the budget is cut to what fits; if not even its first request fits while
+// other lanes run, it waits for them, and with the device idle the retired
+// and idle workspaces are freed first.  A request that alone needs more than
+// the budget completes at once with ZQ_ERR_BUDGET.  Each lane has its own
 // thread that enqueues its batch (a large batch is thousands of launches),
 // so lanes never wait for each other's host work.  A
 // lane is a context with its own stream and resident workspace, so batches on
@@ -26,6 +37,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <list>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -33,6 +45,10 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// pinned descriptor staging each lane allocates ahead (per buffer): the task
+// tables of a batch of 128 problems of 2n <= 4096 take well under this
+constexpr size_t kStageReserve = size_t(16) << 20;
 
 struct Req {
   unsigned long long ticket;
@@ -56,7 +72,11 @@ struct zq_server {
   int device = 0;
   int max_batch = 64;
   size_t max_bytes = 0;
+  size_t budget = 0;                 // device workspace bytes all lanes together may hold
+  size_t default_budget = 0;
   std::vector<zq_context*> lanes;
+  std::vector<size_t> lane_ws;       // workspace bytes each lane holds (or is about to)
+  size_t retired = 0;                // replaced workspaces not freed yet (all lanes)
   std::vector<hipEvent_t> lane_ev;
   std::vector<int*> lane_info;       // pinned, max_batch ints per lane
   std::vector<int> lane_busy;
@@ -69,7 +89,7 @@ struct zq_server {
   std::condition_variable cv_work;   // scheduler: requests or a free lane
   std::condition_variable cv_fly;    // completion thread: a batch in flight
   std::condition_variable cv_done;   // drain
-  std::deque<Req> pending;
+  std::list<Req> pending;            // a list: erasing a batch's members keeps the other iterators valid
   std::deque<Batch> inflight;
   std::deque<zq_completion> done;
   unsigned long long next_ticket = 1;
@@ -89,20 +109,67 @@ struct zq_server {
     return -1;
   }
 
+  // Workspace a batch of the first k requests of n2s needs (grows with k).
+  size_t need_of(const std::vector<int>& n2s, int k, int jobz) const {
+    return zq::batched_workspace(lanes[0], k, n2s.data(), jobz);
+  }
+  // Largest k in [0, kmax] with need_of(k) <= cap (0 if even one does not fit).
+  int fit_prefix(const std::vector<int>& n2s, int kmax, int jobz, size_t cap) const {
+    int lo = 0, hi = kmax;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (need_of(n2s, mid, jobz) <= cap) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+
+  void complete_now(const Req& r, int info) {
+    const double t = now_us();
+    done.push_back(zq_completion{r.ticket, info, 0, -1, r.t_submit, t, t});
+    --outstanding;
+    cv_done.notify_all();
+  }
+
+  bool all_idle() const {
+    for (int b : lane_busy)
+      if (b) return false;
+    return true;
+  }
+
+  // With no work in flight: free every retired workspace, and the workspaces
+  // of the lanes other than `keep` (-1: of every lane).
+  void free_idle_memory(int keep) {
+    for (size_t q = 0; q < lanes.size(); ++q) {
+      (void)zq::context_free_retired(lanes[q]);
+      if ((int)q != keep && lane_ws[q] > 0) {
+        (void)zq::context_release_workspace(lanes[q]);
+        lane_ws[q] = 0;
+      }
+    }
+    retired = 0;
+  }
+
   void run_scheduler() {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
       cv_work.wait(lk, [&] { return stop || (!pending.empty() && free_lane() >= 0); });
+      if (retired > 0 && all_idle()) {
+        for (zq_context* c : lanes) (void)zq::context_free_retired(c);
+        retired = 0;
+      }
       if (pending.empty() || free_lane() < 0) {
         if (stop) break;
         continue;
       }
-      const int L = free_lane();
       // the head request picks the batch; large-class batches may hold at
       // most lanes - 1 lanes, so one lane always stays open to small requests
-      int big_busy = 0;
-      for (int c : lane_cls) big_busy += c >= 2 ? 1 : 0;
+      int busy = 0, big_busy = 0;
+      for (size_t k = 0; k < lanes.size(); ++k) {
+        busy += lane_busy[k];
+        big_busy += lane_cls[k] >= 2 ? 1 : 0;
+      }
       auto head = pending.begin();
       if (lanes.size() > 1 && big_busy + 1 >= (int)lanes.size()) {
         while (head != pending.end() && size_class(head->n2) >= 2) ++head;
@@ -112,27 +179,61 @@ struct zq_server {
           continue;
         }
       }
+      const int jobz = head->jobz;
+      const int cls = size_class(head->n2);
+      std::vector<std::list<Req>::iterator> cand;
+      std::vector<int> n2s;
+      for (auto it = head; it != pending.end() && (int)cand.size() < max_batch; ++it) {
+        if (it->jobz != jobz || size_class(it->n2) != cls) continue;
+        cand.push_back(it);
+        n2s.push_back(it->n2);
+      }
+      if (need_of(n2s, 1, jobz) > budget) {   // can never be served
+        complete_now(*head, ZQ_ERR_BUDGET);
+        pending.erase(head);
+        continue;
+      }
+      int k = (int)cand.size();
+      if (max_bytes > 0 && k > 1) k = std::max(1, fit_prefix(n2s, k, jobz, max_bytes));
+      // the free lane holding the largest workspace
+      int L = -1;
+      size_t total = retired;
+      for (size_t q = 0; q < lanes.size(); ++q) {
+        total += lane_ws[q];
+        if (!lane_busy[q] && (L < 0 || lane_ws[q] > lane_ws[L])) L = (int)q;
+      }
+      if (need_of(n2s, k, jobz) > lane_ws[L]) {
+        // the lane grows: the new workspace is allocated beside the retired old one
+        int k2 = total < budget ? fit_prefix(n2s, k, jobz, budget - total) : 0;
+        if (k2 == 0 && busy == 0) {
+          // the device is idle: free every workspace, the lane's own included
+          // (it is about to be replaced), and the retired ones
+          free_idle_memory(-1);
+          k2 = std::max(1, fit_prefix(n2s, k, jobz, budget));
+        }
+        if (k2 == 0) {
+          // no room to grow beside the running batches: take what the lane holds
+          k2 = fit_prefix(n2s, k, jobz, lane_ws[L]);
+          if (k2 == 0) {
+            // even the head needs more: wait for the running batches (nothing
+            // else is dispatched meanwhile, so the lanes drain)
+            cv_work.wait(lk, [&] { return stop || all_idle(); });
+            continue;
+          }
+        }
+        k = k2;
+      }
+      if (need_of(n2s, k, jobz) > lane_ws[L]) {
+        retired += lane_ws[L];
+        lane_ws[L] = need_of(n2s, k, jobz);
+      }
       lane_busy[L] = 1;
       Batch b;
       b.lane = L;
-      const int jobz = head->jobz;
-      const int cls = size_class(head->n2);
       b.cls = cls;
-      std::vector<int> n2s;
-      for (auto it = head; it != pending.end() && (int)b.reqs.size() < max_batch;) {
-        if (it->jobz != jobz || size_class(it->n2) != cls) {
-          ++it;
-          continue;
-        }
-        n2s.push_back(it->n2);
-        if (!b.reqs.empty() && max_bytes > 0 &&
-            zq::batched_workspace(lanes[L], (int)n2s.size(), n2s.data(), jobz) > max_bytes) {
-          n2s.pop_back();
-          break;
-        }
-        b.reqs.push_back(*it);
-        it = pending.erase(it);
-      }
+      b.reqs.reserve(k);
+      for (int q = 0; q < k; ++q) b.reqs.push_back(*cand[q]);
+      for (int q = 0; q < k; ++q) pending.erase(cand[q]);
       lane_cls[L] = cls;
       lane_job[L] = std::move(b);
       lane_has_job[L] = 1;
@@ -208,7 +309,7 @@ struct zq_server {
         const Req& r = b.reqs[k];
         zq_completion c;
         c.ticket = r.ticket;
-        c.info = b.rc != 0 ? b.rc : info[k];
+        c.info = b.rc != 0 ? b.rc : info_code(info[k]);
         c.batch_size = (int)b.reqs.size();
         c.lane = b.lane;
         c.t_submit_us = r.t_submit;
@@ -219,7 +320,7 @@ struct zq_server {
       outstanding -= (long)b.reqs.size();
       lane_busy[b.lane] = 0;
       lane_cls[b.lane] = -1;
-      cv_work.notify_one();
+      cv_work.notify_all();
       cv_done.notify_all();
     }
   }
@@ -233,13 +334,19 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
   if (lanes < 1 || lanes > 8) return -3;
   if (max_batch < 1) return -4;
   if (max_batch_gb < 0.0) return -5;
+  if (hipSetDevice(device) != hipSuccess) return ZQ_ERR_HIP;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return ZQ_ERR_HIP;
   zq_server* s = new zq_server();
   s->device = device;
   s->max_batch = max_batch;
   s->max_bytes = (size_t)(max_batch_gb * 1073741824.0);
+  s->default_budget = (size_t)(0.8 * (double)fr);
+  s->budget = s->default_budget;
   s->t0 = Clock::now();
   try {   // the per-lane containers, sized once so that the loop below cannot throw
     s->lanes.reserve(lanes);
+    s->lane_ws.reserve(lanes);
     s->lane_ev.reserve(lanes);
     s->lane_info.reserve(lanes);
     s->lane_busy.reserve(lanes);
@@ -257,6 +364,9 @@ int zq_server_create(zq_server** out, int device, int lanes, int max_batch, doub
     rc = zq_create(&c, device);
     if (rc) break;
     s->lanes.push_back(c);
+    s->lane_ws.push_back(0);
+    zq::context_set_defer_free(c, true);
+    if ((rc = zq::context_reserve(c, max_batch, kStageReserve))) break;
     hipEvent_t e = nullptr;
     void* h = nullptr;
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
@@ -350,6 +460,21 @@ int zq_server_drain(zq_server* s) {
 }
 
 double zq_server_clock_us(zq_server* s) { return s ? s->now_us() : 0.0; }
+
+int zq_server_set_budget(zq_server* s, double budget_gb) {
+  if (!s) return ZQ_ERR_ARG;
+  if (budget_gb < 0.0) return -2;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->budget = budget_gb > 0.0 ? (size_t)(budget_gb * 1073741824.0) : s->default_budget;
+  s->cv_work.notify_all();
+  return ZQ_OK;
+}
+
+double zq_server_budget_gb(zq_server* s) {
+  if (!s) return 0.0;
+  std::lock_guard<std::mutex> lk(s->mu);
+  return (double)s->budget / 1073741824.0;
+}
 
 int zq_server_destroy(zq_server* s) {
   if (!s) return ZQ_OK;

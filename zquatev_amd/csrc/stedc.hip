@@ -39,28 +39,42 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // --------------------------------------------------------------------------
 // scale: s = max(max|d|, max|e|) per problem; d, e scaled by 1/s into lam, es.
-// probs holds (offset, size) per problem, one workgroup each.
+// probs holds (offset, size) per problem, one workgroup each.  A tridiagonal
+// holding a NaN or an infinity (an input entry of that kind, or an overflow
+// in the reduction) is replaced by zeros and marks the problem's status word:
+// the merges rank values by comparisons, which a NaN would make inconsistent.
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_dc_scale(const int* __restrict__ probs, const double* __restrict__ d,
                                                    const double* __restrict__ e, double* __restrict__ lam,
-                                                   double* __restrict__ es, double* __restrict__ scal) {
+                                                   double* __restrict__ es, double* __restrict__ scal,
+                                                   int* __restrict__ info) {
   const int o = probs[2 * blockIdx.x], n = probs[2 * blockIdx.x + 1];
   d += o; e += o; lam += o; es += o;
   __shared__ double sh[16];
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(d[i]));
-  for (int i = threadIdx.x; i < n - 1; i += 1024) m = fmax(m, fabs(e[i]));
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    m = fmax(m, fabs(d[i]));
+    bad |= !isfinite(d[i]);
+  }
+  for (int i = threadIdx.x; i < n - 1; i += 1024) {
+    m = fmax(m, fabs(e[i]));
+    bad |= !isfinite(e[i]);
+  }
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) m = fmax(m, __shfl_xor(m, k, 64));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
+  bad = __syncthreads_or(bad);
   double s = 0.0;
   for (int k = 0; k < 16; ++k) s = fmax(s, sh[k]);
-  if (!(s > 0.0)) s = 1.0;
+  if (!(s > 0.0) || bad) s = 1.0;
   const double inv = 1.0 / s;
-  for (int i = threadIdx.x; i < n; i += 1024) lam[i] = d[i] * inv;
-  for (int i = threadIdx.x; i < n - 1; i += 1024) es[i] = fabs(e[i]) * inv;
-  if (threadIdx.x == 0) scal[blockIdx.x] = s;
+  for (int i = threadIdx.x; i < n; i += 1024) lam[i] = bad ? 0.0 : d[i] * inv;
+  for (int i = threadIdx.x; i < n - 1; i += 1024) es[i] = bad ? 0.0 : fabs(e[i]) * inv;
+  if (threadIdx.x == 0) {
+    scal[blockIdx.x] = s;
+    if (bad) atomicOr(info + blockIdx.x, ZQI_INFO_NONFINITE);
+  }
 }
 
 // subtract the coupling beta = es[s-1] from the diagonal entries beside each split s
@@ -94,7 +108,7 @@ __device__ __forceinline__ double lane_get(double v, int i) {
 // of the leaf's eigenvectors (in LDS), to which it applies each rotation.
 __global__ void __launch_bounds__(64) k_dc_leaf(const int* __restrict__ leaves, double* __restrict__ lam,
                                                 const double* __restrict__ es, double* __restrict__ Z, int ldz,
-                                                int* __restrict__ info) {
+                                                int* __restrict__ info, int cap) {
   __shared__ double zz[LEAF][LEAF + 1];
   const int lo = leaves[4 * blockIdx.x], hi = leaves[4 * blockIdx.x + 1];
   const int ob = leaves[4 * blockIdx.x + 2], pr = leaves[4 * blockIdx.x + 3];
@@ -116,7 +130,7 @@ __global__ void __launch_bounds__(64) k_dc_leaf(const int* __restrict__ leaves, 
         if (fabs(lane_get(ek, mm)) <= DEPS * s) break;
       }
       if (mm == l) break;
-      if (++iter > 60) { fail = 1; break; }
+      if (++iter > cap) { fail = 1; break; }
       const double el = lane_get(ek, l);
       double g = (lane_get(dk, l + 1) - lane_get(dk, l)) / (2.0 * el);
       double r = hypot(g, 1.0);
@@ -210,6 +224,8 @@ struct MergeBufs {
   double* Qtmp;
   double* U;
   int lq;        // the merge of rows [lo, hi) keeps its Qtmp and U blocks at lo * lq
+  int* info;     // status word per problem (non-convergence of a secular root)
+  int cap;       // secular-equation steps per root
 };
 
 __device__ __forceinline__ int count_less(const double* a, int len, double v) {
@@ -527,7 +543,8 @@ __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restri
   const double zl2 = rho * z[plo] * z[plo];
   const double zh2 = (phi >= 0) ? rho * z[phi] * z[phi] : 0.0;
   double t = (t0 > a && t0 < b) ? t0 : 0.5 * (a + b);
-  for (int it = 0; it < 200; ++it) {
+  bool conv = false;
+  for (int it = 0; it < B.cap; ++it) {
     const SecSums S = sec_rest(d, z, K, dorg, rho, t, plo, phi);
     const double one = 1.0 + S.r;
     double G, Gp;
@@ -540,13 +557,16 @@ __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restri
       G = tl * one - zl2;
       Gp = one + tl * S.rp;
     }
-    if (G == 0.0) { a = b = t; break; }
+    if (G == 0.0) { a = b = t; conv = true; break; }
     if (G < 0.0) a = t; else b = t;
     double tn = t - G / Gp;
     if (!(tn > a && tn < b)) tn = 0.5 * (a + b);
     const double step = fabs(tn - t);
     t = tn;
-    if (!(b > a) || step <= 4.0 * DEPS * fabs(t) || !(0.5 * (a + b) > a && 0.5 * (a + b) < b)) break;
+    if (!(b > a) || step <= 4.0 * DEPS * fabs(t) || !(0.5 * (a + b) > a && 0.5 * (a + b) < b)) {
+      conv = true;
+      break;
+    }
   }
   // never return the pole itself
   if (org == i && !(t > 0.0)) t = b > 0.0 ? b : DBL_MIN;
@@ -554,6 +574,8 @@ __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restri
   if (lane == 0) {
     B.org[lo + i] = org;
     B.tau[lo + i] = t;
+    // no convergence within the step cap: reported as zhbev reports a QL failure (info > 0)
+    if (!conv) atomicMax(B.info + md.pr, lo - md.ob + i + 1);
   }
 }
 
@@ -656,11 +678,13 @@ __global__ void __launch_bounds__(256) k_merge_copydefl(const MergeDesc* __restr
   Z[(lo + r) + (size_t)(lo - md.ob + B.posD[lo + t]) * ldz] = Q[r + (size_t)(K + t) * ns];
 }
 
+// Eigenvalues of a problem whose input held a NaN or an infinity are NaN.
 __global__ void k_dc_unscale(const int* __restrict__ probs, const double* __restrict__ lam,
-                             const double* __restrict__ scal, double* __restrict__ w) {
+                             const double* __restrict__ scal, const int* __restrict__ info, double* __restrict__ w) {
   const int o = probs[2 * blockIdx.y], n = probs[2 * blockIdx.y + 1];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) w[o + i] = lam[o + i] * scal[blockIdx.y];
+  const bool bad = (info[blockIdx.y] & ZQI_INFO_NONFINITE) != 0;
+  if (i < n) w[o + i] = bad ? __builtin_nan("") : lam[o + i] * scal[blockIdx.y];
 }
 
 // --------------------------------------------------------------------------
@@ -710,7 +734,8 @@ __device__ __forceinline__ int sturm_count(int n, const double* d, const double*
 template <int MS_LANES>
 __global__ void __launch_bounds__(256) k_multisect(const int* __restrict__ probs, const double* __restrict__ d,
                                                    const double* __restrict__ e2, double gl, double gu, double pivmin,
-                                                   const double* __restrict__ scal, double* __restrict__ w) {
+                                                   const double* __restrict__ scal, int* __restrict__ info,
+                                                   double* __restrict__ w, int cap) {
   const int o = probs[2 * blockIdx.y], n = probs[2 * blockIdx.y + 1];
   const int lane = threadIdx.x & 63;
   const int grp = lane / MS_LANES, gl0 = grp * MS_LANES, li = lane - gl0;
@@ -723,7 +748,7 @@ __global__ void __launch_bounds__(256) k_multisect(const int* __restrict__ probs
   // bracket width down to a few ulps of the eigenvalue, or of the matrix
   // norm (1 after scaling) for eigenvalues near zero: an absolute error of
   // eps ||T|| is what any backward-stable eigensolver commits
-  for (int it = 0; it < 64; ++it) {
+  for (int it = 0; it < cap; ++it) {
     if (live && b - a <= 2.0 * DEPS * fmax(fmax(fabs(a), fabs(b)), 1.0)) live = false;
     if (!__any(live)) break;
     const double x = a + (b - a) * ((double)(li + 1) * (1.0 / (MS_LANES + 1)));
@@ -741,12 +766,32 @@ __global__ void __launch_bounds__(256) k_multisect(const int* __restrict__ probs
       b = nbd;
     }
   }
-  if (li == 0 && k < n) w[k] = 0.5 * (a + b) * scal[blockIdx.y];
+  if (live && b - a <= 2.0 * DEPS * fmax(fmax(fabs(a), fabs(b)), 1.0)) live = false;   // narrowed by the last step
+  if (li == 0 && k < n) {
+    const bool bad = (info[blockIdx.y] & ZQI_INFO_NONFINITE) != 0;
+    w[k] = bad ? __builtin_nan("") : 0.5 * (a + b) * scal[blockIdx.y];
+    if (live) atomicMax(info + blockIdx.y, k + 1);   // bracket still open after the step cap
+  }
 }
 
 __global__ void k_square(int n, const double* __restrict__ es, double* __restrict__ e2) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n - 1) e2[i] = es[i] * es[i];
+}
+
+// Eigenvalues only, after k_dc_scale: Sturm-count multisection on every
+// scaled tridiagonal (their Gershgorin intervals lie in [-3, 3]).
+static int values_kernels(int nprob, int nmax, int ntot, const int* iprob, double* w, const StedcWork& sw,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_square, dim3(cdiv(ntot, 256)), dim3(256), 0, s, ntot, sw.es, sw.e2);
+  const double pivmin = DBL_MIN * 4.0;
+  if (nmax <= 2560)
+    hipLaunchKernelGGL(k_multisect<64>, dim3(cdiv(nmax, 4), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
+                       -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, sw.info, w, sw.cap_bisect);
+  else
+    hipLaunchKernelGGL(k_multisect<16>, dim3(cdiv(nmax, 16), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
+                       -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, sw.info, w, sw.cap_bisect);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
 // --------------------------------------------------------------------------
@@ -758,16 +803,16 @@ static void build_tree(int lo, int hi, int ob, int pr, int depth, std::vector<in
   }
   const int mid = lo + (hi - lo) / 2;
   if ((int)lv.size() <= depth) lv.resize(depth + 1);
-  lv[depth].push_back(MergeDesc{lo, mid, hi, ob});
+  lv[depth].push_back(MergeDesc{lo, mid, hi, ob, pr});
   splits.push_back(mid);
   build_tree(lo, mid, ob, pr, depth + 1, leaves, lv, splits);
   build_tree(mid, hi, ob, pr, depth + 1, leaves, lv, splits);
 }
 
 size_t stedc_int_slots(int ntot, int nprob) {
-  // per problem of size n at most 2 ceil(n / LEAF) leaves (4 ints each), one
-  // split and one merge (4 ints) fewer, and (offset, size)
-  return (size_t)18 * (ntot / LEAF + nprob) + 2 * (size_t)nprob + 64;
+  // per problem of size n at most L = 2 ceil(n / LEAF) leaves (4 ints each),
+  // L - 1 splits (1 int) and L - 1 merges (5 ints), and (offset, size)
+  return (size_t)22 * (ntot / LEAF + nprob) + 2 * (size_t)nprob + 64;
 }
 
 int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const double* d, const double* e, double* w,
@@ -793,9 +838,16 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     level_off.push_back(flat.size());
     for (auto& md : levels[dpt]) flat.push_back(md);
   }
-  std::vector<int> host(4 * flat.size() + leaves.size() + splits.size() + probs.size());
+  static_assert(sizeof(MergeDesc) == 5 * sizeof(int), "MergeDesc is uploaded as 5 ints");
+  std::vector<int> host(5 * flat.size() + leaves.size() + splits.size() + probs.size());
   size_t o = 0;
-  for (auto& md : flat) { host[o++] = md.lo; host[o++] = md.mid; host[o++] = md.hi; host[o++] = md.ob; }
+  for (auto& md : flat) {
+    host[o++] = md.lo;
+    host[o++] = md.mid;
+    host[o++] = md.hi;
+    host[o++] = md.ob;
+    host[o++] = md.pr;
+  }
   const size_t o_leaf = o;
   for (int v : leaves) host[o++] = v;
   const size_t o_split = o;
@@ -810,28 +862,17 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
   const int* isplit = sw.ints + o_split;
   const int* iprob = sw.ints + o_prob;
 
-  hipLaunchKernelGGL(k_dc_scale, dim3(nprob), dim3(1024), 0, s, iprob, d, e, sw.lam, sw.es, sw.scal);
-  if (!vectors) {
-    // Gershgorin interval of every scaled matrix lies in [-3, 3]
-    hipLaunchKernelGGL(k_square, dim3(cdiv(ntot, 256)), dim3(256), 0, s, ntot, sw.es, sw.e2);
-    const double pivmin = DBL_MIN * 4.0;
-    if (nmax <= 2560)
-      hipLaunchKernelGGL(k_multisect<64>, dim3(cdiv(nmax, 4), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
-                         -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
-    else
-      hipLaunchKernelGGL(k_multisect<16>, dim3(cdiv(nmax, 16), nprob), dim3(256), 0, s, iprob, sw.lam, sw.e2,
-                         -3.0 - 1e-12, 3.0 + 1e-12, pivmin, sw.scal, w);
-    return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
-  }
+  hipLaunchKernelGGL(k_dc_scale, dim3(nprob), dim3(1024), 0, s, iprob, d, e, sw.lam, sw.es, sw.scal, sw.info);
+  if (!vectors) return values_kernels(nprob, nmax, ntot, iprob, w, sw, s);
   const int nleaf = (int)leaves.size() / 4;
   const int ldz = ntot;
   if ((rc = fill_zero(sw.Z, (size_t)ntot * lq * sizeof(double), s))) return rc;
   if (!splits.empty())
     hipLaunchKernelGGL(k_dc_tear, dim3(cdiv((int)splits.size(), 256)), dim3(256), 0, s, (int)splits.size(), isplit,
                        sw.es, sw.lam);
-  hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, ileaf, sw.lam, sw.es, sw.Z, ldz, sw.info);
+  hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, ileaf, sw.lam, sw.es, sw.Z, ldz, sw.info, sw.cap_leaf);
   MergeBufs B{sw.dk, sw.zk, sw.src, sw.keep, sw.dfl, sw.rot, sw.nrot, sw.kcnt, sw.dkk, sw.zkk, sw.dval, sw.rho,
-              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U, lq};
+              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U, lq, sw.info, sw.cap_secular};
   for (size_t li = 0; li < level_off.size(); ++li) {
     const size_t off = level_off[li];
     const size_t cnt = (li + 1 < level_off.size() ? level_off[li + 1] : flat.size()) - off;
@@ -854,7 +895,7 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     if (rc) return rc;
     hipLaunchKernelGGL(k_merge_copydefl, dim3(cdiv(nsmax, 256), nsmax, nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
   }
-  hipLaunchKernelGGL(k_dc_unscale, dim3(cdiv(nmax, 256), nprob), dim3(256), 0, s, iprob, sw.lam, sw.scal, w);
+  hipLaunchKernelGGL(k_dc_unscale, dim3(cdiv(nmax, 256), nprob), dim3(256), 0, s, iprob, sw.lam, sw.scal, sw.info, w);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

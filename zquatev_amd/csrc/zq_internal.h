@@ -226,6 +226,15 @@ __device__ __forceinline__ quat wave_sum(quat v) {
 #define ZQI_ERR_NOMEM (-1002)
 #define ZQI_ERR_ARG (-1003)
 #define ZQI_ERR_GRAPH (-1004)
+#define ZQI_ERR_NONFINITE (-1005)
+// Per-problem device status word: the low bits hold a positive
+// non-convergence code (atomicMax), this bit marks a NaN / infinity met in
+// the input or in the reduced tridiagonal (atomicOr).  info_code() turns a
+// word into the code the entry points return.
+#define ZQI_INFO_NONFINITE (1 << 30)
+__host__ __device__ inline int info_code(int word) {
+  return (word & ZQI_INFO_NONFINITE) ? ZQI_ERR_NONFINITE : word;
+}
 
 #define ZQ_HIP_CHECK(expr)                                                                  \
   do {                                                                                      \
@@ -297,6 +306,7 @@ struct ReduceWork {
   double* QF;     // per-tile partials of x^* M x: ceil(n/64) (ceil(n/64) + 1)
   double* part;   // ceil(n/16)
   double* scal;   // 16
+  int tsm = 768;  // matvec strip switch: narrow strips up to this trailing size (0 .. 2048)
 };
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s);
 struct QGemmTask;
@@ -316,7 +326,7 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
 // A merge of global rows [lo, hi) with split mid belongs to the problem with
 // base ob.
 struct MergeDesc {
-  int lo, mid, hi, ob;
+  int lo, mid, hi, ob, pr;   // rows [lo, hi) split at mid; problem base row ob and index pr
 };
 struct Staging;
 struct StedcWork {
@@ -328,10 +338,14 @@ struct StedcWork {
   double* scal;   // 16
   int *src, *keep, *dfl, *org, *posK, *posD;                             // n each
   int *nrot, *kcnt;                                                      // n each
-  int* info;      // one per problem: leaf QL failure (first failing local row + 1)
+  int* info;      // one status word per problem (ZQI_INFO_NONFINITE | non-convergence: failing local row + 1)
   int* ints;      // tree descriptors
   size_t int_slots;
   Staging* stage; // host staging of the descriptors
+  // iteration caps: QL sweeps per leaf eigenvalue, secular-equation steps
+  // per root, multisection steps per eigenvalue (reaching one is reported
+  // as non-convergence, info > 0)
+  int cap_leaf = 60, cap_secular = 200, cap_bisect = 64;
 };
 size_t stedc_int_slots(int ntot, int nprob);
 // offs/ns (host): problem offsets and sizes; ntot = sum of ns; lq >= max ns.
@@ -406,7 +420,10 @@ struct BatchProb {
   int off;
   int idx;   // position in the caller's arrays
 };
-int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, hipStream_t s);
+// The loads replace a NaN or infinite input entry by 0 and set
+// ZQI_INFO_NONFINITE in the problem's status word (info[z]), so that every
+// later kernel runs on finite data.
+int load_left_batched(int nprob, int L, const BatchProb* P, quat* M, int ld, ZB zb, int* info, hipStream_t s);
 int store_vectors_batched(int nprob, int L, const BatchProb* P, const quat* X, int ldx, ZB zb, hipStream_t s);
 // dall[off + i] = d_z[i], eall[off + i] = |e_z[i]|
 int gather_tridiag_batched(int nprob, int L, const BatchProb* P, const double* d, const quat* e, ZB zb, double* dall,
@@ -414,7 +431,7 @@ int gather_tridiag_batched(int nprob, int L, const BatchProb* P, const double* d
 int scatter_eig_batched(int nprob, int L, const BatchProb* P, const double* wall, hipStream_t s);
 // info_all[P[z].idx] = info[z]
 int scatter_info_batched(int nprob, const BatchProb* P, const int* info, int* info_all, hipStream_t s);
-int load_left_half(int n, const double* D, int ld2, quat* M, int ld, hipStream_t s);
+int load_left_half(int n, const double* D, int ld2, quat* M, int ld, int* info, hipStream_t s);
 int store_vectors(int n, const quat* X, int ldx, double* D, int ld2, hipStream_t s);
 int offdiag_abs(int n, const quat* e, double* ea, hipStream_t s);
 // zero `bytes` (a multiple of 4) at p with a kernel (no memset node in graphs)
@@ -457,6 +474,16 @@ struct QGemmTask {
 // computes exactly what qgemm would for it.
 int qgemm_tasks(const QGemmTask* tasks_dev, int ntasks, int maxm, int maxnn, bool a_conj_trans, bool b_conj_trans,
                 bool lower_only, hipStream_t s);
+// GEMM tile rows: 64-row tiles for launches of at least `g_tile64_per_cu`
+// 64 x 64 tiles per compute unit, else 32-row tiles.  A tuning parameter of
+// the calling thread (set per solve from its context; 16 by default): the
+// tile shape does not change any result, only the time.
+extern thread_local int g_tile64_per_cu;
+struct TileScope {
+  int saved;
+  explicit TileScope(int v) : saved(g_tile64_per_cu) { if (v > 0) g_tile64_per_cu = v; }
+  ~TileScope() { g_tile64_per_cu = saved; }
+};
 // number of K chunks (and the chunk length) a split-K product over K uses
 int splitk_count(int K, int nsplit, int* kchunk_out);
 // real merge products of one divide & conquer tree level (see stedc.hip)
@@ -476,4 +503,15 @@ int batched_async(zq_context* c, int count, const int* n2, double* const* D, con
 // device workspace bytes such a batch needs on c
 size_t batched_workspace(const zq_context* c, int count, const int* n2, int jobz);
 hipStream_t context_stream(const zq_context* c);
+// Allocates ahead what a batch of up to max_count problems needs besides the
+// workspace (status words, pinned descriptor staging of stage_bytes per
+// buffer), so that later batches never free and reallocate them.
+int context_reserve(zq_context* c, int max_count, size_t stage_bytes);
+// Frees the workspace (the context must have no work in flight).
+int context_release_workspace(zq_context* c);
+// With defer-free on, a workspace replaced by a larger one is retired rather
+// than freed (hipFree waits for the whole device); context_free_retired frees
+// the retired ones (call while no work is in flight on the device).
+void context_set_defer_free(zq_context* c, bool on);
+int context_free_retired(zq_context* c);
 }  // namespace zq
