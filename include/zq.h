@@ -248,6 +248,28 @@ int zq_instrument_report(zq_context* ctx, int cls, double* total_ms, int* count,
  * A and B (64 doubles each, device) as the per-lane operands. */
 int zq_selftest_mfma(zq_context* ctx, const double* dA, const double* dB, double* dOut);
 
+/* ---- An entry point that allocates nothing ----------------------------------
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation.  Eigenvalues only -- the kernels of
+ * zq_zquatev_device with jobz = 'N' and the built-in panel width, so the
+ * result equals that call's on a context with no panel-width override or
+ * autotuning -- of one problem of order n2 <= ZQ_NOALLOC_MAX_N2 (dD, nld2,
+ * deig as there; dD is only read), on the current HIP device and the given
+ * stream (a hipStream_t as void*, NULL for the null stream; asynchronous).
+ * Every scratch buffer lies in one fixed device array of
+ * ZQ_NOALLOC_SCRATCH_BYTES compiled into the library (each device the
+ * library runs on holds it until the process ends); the call allocates
+ * nothing and copies nothing from the host.  All calls on one device share
+ * that array and leave their intermediate data in it: calls that overlap on
+ * one device race -- serialise them (one stream, or events).  dinfo (device
+ * int, may be NULL) receives the status zq_last_info would report (0, > 0
+ * no convergence, ZQ_ERR_NONFINITE).  Returns 0, -1..-4 as zq_zquatev, -5 for
+ * n2 above the maximum, ZQ_ERR_HIP. */
+#define ZQ_NOALLOC_MAX_N2 2048
+#define ZQ_NOALLOC_SCRATCH_BYTES (48u << 20)
+int zq_eigvals_noalloc(int n2, const double* dD, int nld2, double* deig, int* dinfo, void* stream);
+
 /* Stream synchronisation helper for callers without their own HIP runtime binding. */
 int zq_synchronize(zq_context* ctx);
 

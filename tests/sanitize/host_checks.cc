@@ -34,6 +34,8 @@ int main() {
   CHECK(zq_zquatev(0, nullptr, 1, nullptr) == 0);
   CHECK(ts::zquatev(5, D.data(), 4, eig.data()) == -1);
   CHECK(zq_zquatev_device(nullptr, 4, nullptr, 4, nullptr, 'V') == ZQ_ERR_ARG);
+  CHECK(zq_eigvals_noalloc(2050, reinterpret_cast<double*>(D.data()), 2050, eig.data(), nullptr, nullptr) == -5);
+  CHECK(zq_eigvals_noalloc(7, reinterpret_cast<double*>(D.data()), 8, eig.data(), nullptr, nullptr) == -1);
   const int one = 4;
   CHECK(zq_zquatev_batched_device(nullptr, 1, &one, nullptr, &one, nullptr, 'V', nullptr) == ZQ_ERR_ARG);
   CHECK(zq_last_info(nullptr) == ZQ_ERR_ARG);

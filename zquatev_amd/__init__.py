@@ -56,6 +56,7 @@ _SIGNATURES = {
     "zq_set_workspace_limit": (_c_int, [_vp, ctypes.c_size_t]),
     "zq_set_iteration_caps": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "zq_batch_workspace": (_c_int, [_vp, _c_int, _vp, _c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _vp]),
+    "zq_eigvals_noalloc": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp]),
     "zq_server_set_budget": (_c_int, [_vp, ctypes.c_double]),
     "zq_server_budget_gb": (ctypes.c_double, [_vp]),
     "zq_set_instrument": (_c_int, [_vp, _c_int]),

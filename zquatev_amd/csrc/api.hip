@@ -748,6 +748,37 @@ int check_args(int n2, const void* D, int nld2, const void* eig) {
 std::mutex g_mu;
 zq_context* g_ctx = nullptr;
 
+// ---- the entry point that allocates nothing ------------------------------
+// Every buffer of zq_eigvals_noalloc lies in this array, one copy per device
+// (held until the process ends).  Host code never uses the name itself --
+// on the host it only stands for the symbol -- but the device address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr size_t kScratchBytes = ZQ_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on the current device (looked up on the first
+// call there), or null on failure.
+void* scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 // ---- host <-> device copies of the host entry points --------------------
@@ -1287,6 +1318,31 @@ int zq_zquatev(int n2, double* D, int nld2, double* eig) try {
     if (rc) return rc;
   }
   return zq_zquatev_ctx(g_ctx, n2, D, nld2, eig);
+} catch (...) {
+  return exception_status();
+}
+
+int zq_eigvals_noalloc(int n2, const double* dD, int nld2, double* deig, int* dinfo, void* stream) try {
+  int rc = check_args(n2, dD, nld2, deig);
+  if (rc) return rc;
+  if (n2 > ZQ_NOALLOC_MAX_N2) return -5;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n2 == 0) return dinfo ? zq::fill_zero(dinfo, sizeof(int), s) : ZQ_OK;
+  const int n = n2 / 2;
+  const int nb = zq_tuned_panel_width(n);
+  Plan p;
+  if (layout_plan(n, nb, false, nullptr, &p) > kScratchBytes) return -5;   // checked before any device call
+  char* base = static_cast<char*>(scratch_on_current_device());
+  if (!base) return ZQ_ERR_HIP;
+  layout_plan(n, nb, false, base, &p);
+  // the kernels of zq_zquatev_device(jobz = 'N') with the built-in panel width
+  if ((rc = zq::fill_zero(p.sw.info, sizeof(int), s))) return rc;   // the status word starts at 0
+  if ((rc = zq::load_left_half(n, dD, nld2, p.M, p.ldm, p.sw.info, s))) return rc;
+  if ((rc = zq::reduce(p.M, n, p.ldm, nb, p.d, p.e, p.rw, s))) return rc;
+  if ((rc = zq::offdiag_abs(n, p.e, p.ea, s))) return rc;
+  if ((rc = zq::eigvals_one(n, p.d, p.ea, deig, p.sw, s))) return rc;
+  if (dinfo && (rc = zq::status_out(p.sw.info, dinfo, s))) return rc;
+  return ZQ_OK;
 } catch (...) {
   return exception_status();
 }

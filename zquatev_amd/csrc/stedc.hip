@@ -794,6 +794,26 @@ static int values_kernels(int nprob, int nmax, int ntot, const int* iprob, doubl
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 
+__global__ void k_set_prob(int* __restrict__ probs, int o, int n) {
+  probs[0] = o;
+  probs[1] = n;
+}
+
+int eigvals_one(int n, const double* d, const double* e, double* w, const StedcWork& sw, hipStream_t s) {
+  if (n <= 0) return ZQI_OK;
+  int* iprob = sw.ints;   // (offset, size) written by a kernel: nothing is uploaded from the host
+  hipLaunchKernelGGL(k_set_prob, dim3(1), dim3(1), 0, s, iprob, 0, n);
+  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(1024), 0, s, iprob, d, e, sw.lam, sw.es, sw.scal, sw.info);
+  return values_kernels(1, n, n, iprob, w, sw, s);
+}
+
+__global__ void k_status_out(const int* __restrict__ word, int* __restrict__ out) { out[0] = info_code(word[0]); }
+
+int status_out(const int* word, int* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_status_out, dim3(1), dim3(1), 0, s, word, out);
+  return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
+}
+
 // --------------------------------------------------------------------------
 static void build_tree(int lo, int hi, int ob, int pr, int depth, std::vector<int>& leaves,
                        std::vector<std::vector<MergeDesc>>& lv, std::vector<int>& splits) {

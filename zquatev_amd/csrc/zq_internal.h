@@ -353,6 +353,12 @@ size_t stedc_int_slots(int ntot, int nprob);
 // w (ntot): eigenvalues ascending per problem.  Z (when vectors) is ld ntot.
 int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const double* d, const double* e, double* w,
           const StedcWork& sw, bool vectors, hipStream_t s);
+// Eigenvalues of one tridiagonal of order n, the kernels stedc runs for it
+// without vectors, with nothing staged from the host (sw.ints holds the
+// problem's (offset, size), written by a kernel).
+int eigvals_one(int n, const double* d, const double* e, double* w, const StedcWork& sw, hipStream_t s);
+// *out = info_code(*word) on the device
+int status_out(const int* word, int* out, hipStream_t s);
 
 // Pinned host staging for host -> device descriptor tables.  Two buffers
 // alternate between solves; stage_begin waits until the solve that last used
