@@ -36,6 +36,11 @@ def parse():
     ap.add_argument("--nb", type=int, default=0,
                     help="panel width (0: autotuned over 16/32/64/128 before the warm-up; the built-in table "
                          "with --profile-run)")
+    ap.add_argument("--tsm", type=int, default=-1,
+                    help="matvec strip switch (trailing rows with narrow strips; -1: autotuned, or the default "
+                         "with --profile-run)")
+    ap.add_argument("--tile64", type=int, default=0,
+                    help="GEMM 64-row tiles from this many 64x64 tiles per CU (0: autotuned / default)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--cpu-n2", type=int, default=4096,
                     help="size of the bounded CPU-baseline sample (median of --cpu-reps runs on the host cores)")
@@ -69,7 +74,8 @@ def parse():
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched summary of the single workload")
     ap.add_argument("--profile-run", action="store_true",
-                    help="only warmup + timed steps (for rocprofv3), no instrumentation pass or baseline")
+                    help="only warmup + timed steps (for rocprofv3), no instrumentation pass or baseline (with "
+                         "--workload batched: no plain-loop baseline either)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-run: also run one elementwise copy (torch.neg) of MIB MiB, "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -191,9 +197,11 @@ def batched_workload(count, seed, dev):
     return n2s, buf, views, buf.clone()
 
 
-def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
+def run_batched(ctx, stream, count, seed, steps, warmup, check=True, profile=False):
     """Batched solves/s of the batched workload against the plain loop of
-    single device solves on the same stream."""
+    single device solves on the same stream.  profile=True (for rocprofv3):
+    only the warm-up and timed batched calls, without the loop baseline, the
+    instrumented batch and the checks."""
     import torch
     from zquatev_amd import gen
     dev = torch.device("cuda:0")
@@ -226,6 +234,10 @@ def run_batched(ctx, stream, count, seed, steps, warmup, check=True):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / k
 
+    if profile:
+        t_batch = timed(batch_step, steps, warmup)
+        return {"problems": count, "batched_ms": round(1000 * t_batch, 2),
+                "batched_solves_per_s": round(count / t_batch, 1)}
     t_loop = timed(loop_step, 1, 1)
     res_loop = [e.clone() for e in eigs[:8]]
     t_batch = timed(batch_step, steps, warmup)
@@ -464,10 +476,11 @@ def run_serving(args, summary=False):
     return 0
 
 
-def measured_traffic(kernel, n2, nb):
+def measured_traffic(kernel, n2, nb, tsm):
     """HBM bytes per launch of `kernel` at this workload from the committed
     counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
-    scaled by a known-size copy), or None when no summary covers it."""
+    scaled by a known-size copy), or None when no summary covers it (another
+    panel width or matvec strip switch)."""
     import re
     path = os.path.join(ROOT, "profiles", f"{kernel}_2n{n2}_pmc.md")
     if not os.path.exists(path):
@@ -475,7 +488,7 @@ def measured_traffic(kernel, n2, nb):
     text = open(path).read()
     if f"panel width {nb}" not in text:
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    m = re.search(r"traffic_bytes_per_launch \(matvec switch %d\):\s*([0-9.eE+]+)" % tsm, text)
     return float(m.group(1)) if m else None
 
 
@@ -500,7 +513,11 @@ def main():
         ctx = zq.Context(0)
         stream = torch.cuda.current_stream(torch.device("cuda:0"))
         ctx.set_stream(stream.cuda_stream)
-        r = run_batched(ctx, stream, args.batch_count, 1002, args.steps, args.warmup, check=not args.no_check)
+        r = run_batched(ctx, stream, args.batch_count, 1002, args.steps, args.warmup, check=not args.no_check,
+                        profile=args.profile_run)
+        if args.profile_run:
+            print(json.dumps(r))
+            return 0
         line = {"metric": "batched solves/s (batched workload)", "value": r["batched_solves_per_s"],
                 "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": r["batched_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -533,6 +550,11 @@ def main():
         nb = chosen[0]
         tune = {str(w): round(float(t), 3) for w, t in zip((16, 32, 64, 128), tms[0]) if t >= 0}
         print(f"autotune n={n}: reduction ms {tune} -> panel width {nb}", file=sys.stderr, flush=True)
+    if args.tsm >= 0 or args.tile64 > 0:
+        ctx.set_tile_shape(args.tsm, args.tile64)
+    tsm, t64 = ctx.tile_shape(n)
+    print(f"tile shape n={n}: matvec switch {tsm}, GEMM 64-row tiles from {t64} tiles/CU", file=sys.stderr,
+          flush=True)
 
     def step():
         Dt[:n, :].copy_(pristine)
@@ -560,7 +582,8 @@ def main():
     value = flops / (ms_per_step / 1000.0) / 1e9
 
     if args.profile_run:
-        out = {"ms_per_step": ms_per_step, "gflops": value, "phases_ms": phases}
+        out = {"ms_per_step": ms_per_step, "gflops": value, "phases_ms": phases, "panel_width": nb,
+               "matvec_switch": tsm, "gemm64_tiles_per_cu": t64}
         if args.calib_copy > 0:
             nel = args.calib_copy * (1 << 20) // 8
             src = torch.ones(nel, dtype=torch.float64, device=dev)
@@ -647,7 +670,7 @@ def main():
     if dom == "symv":
         ach = kd["units"] / (kd["total_ms"] / 1000.0) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_symv", n2, nb),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_symv", n2, nb, tsm),
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                 "algorithmic_bytes_per_launch": kd["units"] / kd["launches"],
                 "kernel": "k_symv (panel Hermitian matvec)"}
@@ -699,7 +722,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "single solve, 2n=8192 complex128, A Hermitian / B antisymmetric U(-1,1), "
                                "eigenvalues + Kramers-paired eigenvectors",
-                   "n2": n2, "panel_width": nb, "panel_width_autotune_ms": tune, "seed": args.seed,
+                   "n2": n2, "panel_width": nb, "panel_width_autotune_ms": tune,
+                   "tile_shape": {"matvec_switch": tsm, "gemm64_tiles_per_cu": t64}, "seed": args.seed,
                    "generator": "splitmix64",
                    "flop_count": "(160/3) n^3, n = 2n/2"},
         "event_ms_per_step": round(ev_ms, 3),

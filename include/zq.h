@@ -48,9 +48,13 @@ int zq_version(void);
 /* Human-readable text for a return code. */
 const char* zq_error_string(int code);
 
-/* Create a context bound to HIP device `device`: one stream (created here) and
- * a device workspace cached across calls.  No counterpart in the reference,
- * whose entry point allocates per call (zquatev.cc:63-66). */
+/* Create a context bound to HIP device `device`: its own stream and a device
+ * workspace cached across calls.  The stream is created when work first runs
+ * on it, so a context that is given a caller's stream (zq_set_stream) never
+ * holds one: each stream takes one of the process's GPU_MAX_HW_QUEUES
+ * hardware queues (4 by default), and streams beyond those share queues and
+ * serialise.  No counterpart in the reference, whose entry point allocates
+ * per call (zquatev.cc:63-66). */
 int zq_create(zq_context** ctx, int device);
 int zq_destroy(zq_context* ctx);
 
@@ -181,6 +185,9 @@ typedef struct {
 } zq_completion;
 
 /* lanes 1..8 (-3), max_batch >= 1 (-4), max_batch_gb >= 0, 0 = no limit (-5).
+ * Each lane is a stream: for its batches to overlap, the process's streams in
+ * use (the lanes plus the caller's own) must not exceed GPU_MAX_HW_QUEUES
+ * (4 by default, so up to 3 lanes beside one caller stream).
  * The lanes' workspaces together stay within a device-memory budget, by
  * default 80% of the device memory free at creation (zq_server_set_budget).
  * A lane's workspace grows only while no other lane has work in flight (a

@@ -37,7 +37,8 @@ static thread_local ProfState* g_prof = nullptr;
 struct zq_context {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t own_stream = nullptr;
+  hipStream_t own_stream = nullptr;   // created on first use (see with_device)
+  bool use_own = true;                // work runs on own_stream rather than a caller's stream
   int nb_override = 0;
   int tuned_nb[32] = {};      // autotuned panel width per bucket floor(log2 n) (0: none)
   // tile shapes (per bucket, autotuned; -1 / 0: the default): the matvec
@@ -416,9 +417,26 @@ void apply_caps(const zq_context* ctx, zq::StedcWork& sw) {
   if (ctx->caps[2] > 0) sw.cap_bisect = ctx->caps[2];
 }
 
+// Makes the context's device current and, when the context runs on its own
+// stream, creates that stream on first use.  A stream takes one of the
+// process's hardware queues (HIP maps streams onto GPU_MAX_HW_QUEUES of them,
+// 4 by default, sharing queues beyond that); a context that only ever runs on
+// a caller's stream never holds one, so it cannot force the streams of other
+// contexts -- the lanes of a server, say -- to share queues and serialise.
 int with_device(zq_context* ctx) {
-  return hipSetDevice(ctx->device) == hipSuccess ? ZQ_OK : ZQ_ERR_HIP;
+  if (hipSetDevice(ctx->device) != hipSuccess) return ZQ_ERR_HIP;
+  if (ctx->use_own) {
+    if (!ctx->own_stream && hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      ctx->own_stream = nullptr;
+      return ZQ_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+  }
+  return ZQ_OK;
 }
+
+// Nothing has been queued on this context yet (its own stream is not even created).
+bool idle_unstarted(const zq_context* ctx) { return ctx->use_own && !ctx->own_stream; }
 
 // The device solve: dD (ld2 complex) -> dD, deig.
 int solve_device(zq_context* ctx, int n, double* dD, int ld2, double* deig, bool vectors) {
@@ -968,11 +986,11 @@ int zq_create(zq_context** out, int device) try {
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return ZQ_ERR_HIP;
   zq_context* c = new zq_context();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess) {
     zq_destroy(c);
     return ZQ_ERR_HIP;
   }
-  c->stream = c->own_stream;
+  c->stream = nullptr;   // the own stream is created on first use
   for (auto& e : c->ev) {
     if (hipEventCreateWithFlags(&e, zq::kTimingEvent) != hipSuccess) {
       zq_destroy(c);
@@ -1042,12 +1060,14 @@ int zq_destroy(zq_context* c) {
 int zq_set_stream(zq_context* c, void* stream) {
   if (!c) return -1;
   c->stream = static_cast<hipStream_t>(stream);
+  c->use_own = false;
   return ZQ_OK;
 }
 
 int zq_use_own_stream(zq_context* c) {
   if (!c) return -1;
-  c->stream = c->own_stream;
+  c->use_own = true;
+  c->stream = c->own_stream;   // null until first use
   return ZQ_OK;
 }
 
@@ -1426,7 +1446,7 @@ int zq_set_instrument(zq_context* c, int on) {
 
 int zq_instrument_report(zq_context* c, int cls, double* total_ms, int* count, double* units) {
   if (!c || cls < 0 || cls >= zq::PROF_NCLASS) return ZQ_ERR_ARG;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
+  if (!idle_unstarted(c) && hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
   double t = 0.0, u = 0.0;
   int k = 0;
   for (size_t i = 0; i < c->prof.used; ++i) {
@@ -1452,11 +1472,13 @@ int zq_selftest_mfma(zq_context* c, const double* dA, const double* dB, double* 
 
 int zq_synchronize(zq_context* c) {
   if (!c) return ZQ_ERR_ARG;
+  if (idle_unstarted(c)) return ZQ_OK;
   return hipStreamSynchronize(c->stream) == hipSuccess ? ZQ_OK : ZQ_ERR_HIP;
 }
 
 int zq_last_info(zq_context* c) try {
   if (!c) return ZQ_ERR_ARG;
+  if (idle_unstarted(c)) return ZQ_OK;
   if (with_device(c)) return ZQ_ERR_HIP;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return ZQ_ERR_HIP;
   if (c->last_kind == 1 && c->info_dev) {

@@ -486,15 +486,19 @@ __device__ __forceinline__ SecSums sec_rest(const double* d, const double* z, in
   return SecSums{rho * r, rho * rp};
 }
 
-__global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restrict__ descs, MergeBufs B) {
-  const MergeDesc md = descs[blockIdx.y];
+// A block serves ROOTS roots of one merge.  Every Newton step re-reads the
+// merge's poles d and weights z, so merges of up to SEC_LDS kept values
+// first stage both in LDS (one read from L2 per block instead of one per
+// root and step; the arithmetic and its order are unchanged).  The LDS is
+// sized for the level's largest merge, and levels of small merges (the many
+// deep merges of a batch) use blocks of 4 roots, so that they keep their
+// occupancy.
+constexpr int SEC_LDS = 4096;
+
+__device__ __forceinline__ void secular_root(const MergeDesc& md, const MergeBufs& B, int K, int i, const double* d,
+                                             const double* z) {
   const int lo = md.lo;
-  const int K = B.kcnt[blockIdx.y];
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= K) return;   // wave-uniform
   const double rho = B.rho[blockIdx.y];
-  const double* d = B.dkk + lo;
-  const double* z = B.zkk + lo;
   const int lane = threadIdx.x & 63;
   int org, plo, phi;
   double a, b, pl, ph;   // bracket (a, b) and pole positions relative to the origin
@@ -576,6 +580,32 @@ __global__ void __launch_bounds__(256) k_merge_secular(const MergeDesc* __restri
     B.tau[lo + i] = t;
     // no convergence within the step cap: reported as zhbev reports a QL failure (info > 0)
     if (!conv) atomicMax(B.info + md.pr, lo - md.ob + i + 1);
+  }
+}
+
+// kst: doubles of dynamic LDS per staged array (the level's largest merge,
+// 0 when it exceeds SEC_LDS and the roots read d and z from global memory)
+template <int ROOTS>
+__global__ void __launch_bounds__(64 * ROOTS) k_merge_secular(const MergeDesc* __restrict__ descs, MergeBufs B,
+                                                             int kst) {
+  const MergeDesc md = descs[blockIdx.y];
+  const int K = B.kcnt[blockIdx.y];
+  if ((int)blockIdx.x * ROOTS >= K) return;   // block-uniform: no root of this merge here
+  const int i = blockIdx.x * ROOTS + (threadIdx.x >> 6);
+  const double* d = B.dkk + md.lo;
+  const double* z = B.zkk + md.lo;
+  if (K <= kst) {   // block-uniform
+    extern __shared__ double sdz[];
+    double* sd = sdz;
+    double* sz = sdz + kst;
+    for (int q = threadIdx.x; q < K; q += 64 * ROOTS) {
+      sd[q] = d[q];
+      sz[q] = z[q];
+    }
+    __syncthreads();
+    if (i < K) secular_root(md, B, K, i, sd, sz);   // wave-uniform
+  } else if (i < K) {
+    secular_root(md, B, K, i, d, z);
   }
 }
 
@@ -904,7 +934,14 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     hipLaunchKernelGGL(k_merge_rot, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
     hipLaunchKernelGGL(k_merge_gather, dim3(cdiv(nsmax, 256), cdiv(nsmax, GATHER_COLS), nm), dim3(256), 0, s, dd,
                        sw.Z, ldz, B);
-    hipLaunchKernelGGL(k_merge_secular, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
+    {
+      const int kst = nsmax <= SEC_LDS ? nsmax : 0;
+      const unsigned shm = (unsigned)(2 * kst * sizeof(double));
+      if (nsmax <= 512)
+        hipLaunchKernelGGL(k_merge_secular<4>, dim3(cdiv(nsmax, 4), nm), dim3(256), shm, s, dd, B, kst);
+      else
+        hipLaunchKernelGGL(k_merge_secular<16>, dim3(cdiv(nsmax, 16), nm), dim3(1024), shm, s, dd, B, kst);
+    }
     hipLaunchKernelGGL(k_merge_zhat, dim3(cdiv(nsmax, 4), nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_buildU, dim3(nsmax, nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_order, dim3(nm), dim3(256), 0, s, dd, sw.lam, B);
