@@ -236,6 +236,18 @@ int zq_tridiagonalize(zq_context* ctx, int n2, const double* dD, int nld2, doubl
  * NULL for eigenvalues only): orthonormal eigenvectors. */
 int zq_tridiag_eig(zq_context* ctx, int n, const double* dd, const double* de, double* dw, double* dZ);
 
+/* zq_qgemm: the quaternion GEMM on FP64 MFMA that runs the trailing update
+ * (blocked.cc:416-475's zgemm3m_ calls) and the back-transform
+ * (zquatev.cc:86-90): C (m x n) = alpha op(A) op(B) + beta C with op(X) = X
+ * (trans 'N') or its quaternion conjugate transpose (trans 'C').  A
+ * quaternion q = (a, b), i.e. the 2 x 2 complex block [[a, -conj b], [b,
+ * conj a]], is stored as the 4 doubles (Re a, Im a, Re b, Im b); leading
+ * dimensions count quaternions; lower_only != 0 writes only C[i][j] with
+ * i >= j.  Device memory, asynchronous.  Returns 0, -2 / -3 bad trans, -4 a
+ * negative size, -5 / -6 a null operand, -7 a leading dimension too small. */
+int zq_qgemm(zq_context* ctx, int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
+             const double* B, int ldb, double beta, double* C, int ldc, int lower_only);
+
 /* Device-side timings of the last solve on ctx, in milliseconds:
  * ms[0] reduction, ms[1] tridiagonal eigensolve, ms[2] back-transform,
  * ms[3] layout in/out, ms[4] total.  Returns the number of values written. */

@@ -49,6 +49,8 @@ _SIGNATURES = {
     "zq_server_destroy": (_c_int, [_vp]),
     "zq_tridiagonalize": (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
     "zq_tridiag_eig": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
+    "zq_qgemm": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _c_int, _vp, _c_int,
+                          ctypes.c_double, _vp, _c_int, _c_int]),
     "zq_last_timings": (_c_int, [_vp, ctypes.POINTER(ctypes.c_double), _c_int]),
     "zq_selftest_mfma": (_c_int, [_vp, _vp, _vp, _vp]),
     "zq_synchronize": (_c_int, [_vp]),
@@ -335,6 +337,14 @@ class Context:
 
     def tridiag_eig_device(self, n, dd_ptr, de_ptr, dw_ptr, dz_ptr=0):
         return _fn("zq_tridiag_eig")(self._h, int(n), _vp(dd_ptr), _vp(de_ptr), _vp(dw_ptr), _vp(dz_ptr or 0))
+
+    def qgemm_device(self, transa, transb, m, n, k, alpha, a_ptr, lda, b_ptr, ldb, beta, c_ptr, ldc,
+                     lower_only=False):
+        """Quaternion GEMM on device buffers of 4 doubles per quaternion:
+        C = alpha op(A) op(B) + beta C, op = 'N' or 'C' (conjugate transpose)."""
+        return _fn("zq_qgemm")(self._h, ord(transa), ord(transb), int(m), int(n), int(k), float(alpha),
+                               _vp(a_ptr), int(lda), _vp(b_ptr), int(ldb), float(beta), _vp(c_ptr), int(ldc),
+                               1 if lower_only else 0)
 
     def selftest_mfma_device(self, a_ptr, b_ptr, out_ptr):
         return _fn("zq_selftest_mfma")(self._h, _vp(a_ptr), _vp(b_ptr), _vp(out_ptr))

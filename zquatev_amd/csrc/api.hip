@@ -13,6 +13,7 @@
 #include "zquatev.h"
 
 #include <algorithm>
+#include <climits>
 #include <complex>
 #include <cstring>
 #include <mutex>
@@ -1390,6 +1391,29 @@ int zq_tridiagonalize(zq_context* c, int n2, const double* dD, int nld2, double*
   if (n > 1 && de)
     ZQ_HIP_CHECK(hipMemcpyAsync(de, p.ea, (size_t)(n - 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
   return ZQ_OK;
+} catch (...) {
+  return exception_status();
+}
+
+int zq_qgemm(zq_context* c, int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
+             const double* B, int ldb, double beta, double* C, int ldc, int lower_only) try {
+  if (!c) return ZQ_ERR_ARG;
+  const bool ta = transa == 'C' || transa == 'c', tb = transb == 'C' || transb == 'c';
+  if (!ta && transa != 'N' && transa != 'n') return -2;
+  if (!tb && transb != 'N' && transb != 'n') return -3;
+  if (m < 0 || n < 0 || k < 0) return -4;
+  if (m == 0 || n == 0) return ZQ_OK;
+  if (k > 0 && (!A || !B)) return -5;
+  if (!C) return -6;
+  if (lda < std::max(1, ta ? k : m) || ldb < std::max(1, tb ? n : k) || ldc < std::max(1, m)) return -7;
+  int rc;
+  if ((rc = with_device(c))) return rc;
+  const quat* qa = reinterpret_cast<const quat*>(A);
+  const quat* qb = reinterpret_cast<const quat*>(B);
+  const zq::QOperand oa{qa, qa, lda, lda, INT_MAX, ta, INT_MIN / 2};
+  const zq::QOperand ob{qb, qb, ldb, ldb, INT_MAX, tb, INT_MIN / 2};
+  // k == 0 runs no K stage: C = beta C
+  return zq::qgemm(m, n, k, alpha, oa, ob, beta, reinterpret_cast<quat*>(C), ldc, lower_only != 0, c->stream);
 } catch (...) {
   return exception_status();
 }
