@@ -520,28 +520,39 @@ __device__ __forceinline__ void secular_root(const MergeDesc& md, const MergeBuf
     phi = i + 1;
     if (1.0 + rho * s >= 0.0) { org = i; a = 0.0; b = half; pl = 0.0; ph = gap; }
     else { org = i + 1; a = -half; b = 0.0; pl = -gap; ph = 0.0; }
-    // First iterate: the root in (0, gap) of the two-pole model
-    //   c + rho z_i^2 / (0 - t) + rho z_{i+1}^2 / (gap - t),  c = 1 + rho (other poles at the midpoint),
-    // i.e. of -c t^2 + (c gap + zl + zh) t - zl gap (the rational initial guess of
-    // LAPACK's dlaed4, in its simplest form).
+    // First iterate: the root in the bracket of the two-pole model
+    //   c + rho z_i^2 / (pl - t) + rho z_{i+1}^2 / (ph - t),  c = 1 + rho (other poles at the midpoint),
+    // i.e. of -c t^2 + (c (pl + ph) + zl + zh) t - zl ph - zh pl (pl ph = 0), written
+    // about the origin so that a root a few ulps from its pole keeps its
+    // relative accuracy (the rational initial guess of LAPACK's dlaed4, in its
+    // simplest form).  The quadratic has one root in (pl, ph); it is the one of
+    // the two stable-formula roots that lies in (a, b), else the midpoint.
     const double zl = rho * z[i] * z[i], zh = rho * z[i + 1] * z[i + 1];
     const double c = 1.0 + rho * sr;
-    const double a1 = c * gap + zl + zh, a0 = -zl * gap;
-    double r = -a0 / a1;
-    if (c != 0.0) {
-      const double disc = fmax(a1 * a1 + 4.0 * c * a0, 0.0);
-      const double qq = -0.5 * (a1 + copysign(sqrt(disc), a1));
-      const double r1 = qq / (-c), r2 = (qq != 0.0) ? a0 / qq : -1.0;
-      r = (r1 > 0.0 && r1 < gap) ? r1 : r2;
-    }
-    t0 = (org == i) ? r : r - gap;
+    const double qa = -c, qb = c * (pl + ph) + zl + zh, qc = -zl * ph - zh * pl;
+    const double disc = fmax(qb * qb - 4.0 * qa * qc, 0.0);
+    const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
+    const double r1 = (qa != 0.0) ? qq / qa : 0.5 * (a + b);
+    const double r2 = (qq != 0.0) ? qc / qq : 0.5 * (a + b);
+    t0 = (r1 > a && r1 < b) ? r1 : r2;
   } else {
     double zz = 0.0;
     for (int jj = lane; jj < K; jj += 64) zz += z[jj] * z[jj];
     zz = wave_sum_d(zz);
     org = i; plo = i; phi = -1;
     a = 0.0; b = rho * zz; pl = 0.0; ph = 0.0;
-    t0 = 0.5 * (a + b);
+    // First iterate: the root zl / c of the one-pole model c + rho z_i^2 / (0 - t),
+    // c = 1 + rho (the other poles at the midpoint of the bracket)
+    const double tm = 0.5 * (a + b);
+    double sr = 0.0;
+    for (int jj = lane; jj < K - 1; jj += 64) {
+      const double zj = z[jj];
+      sr += zj * zj / ((d[jj] - d[i]) - tm);
+    }
+    sr = wave_sum_d(sr);
+    const double c = 1.0 + rho * sr;
+    const double r = rho * z[i] * z[i] / c;
+    t0 = (c > 0.0 && r > a && r < b) ? r : tm;
   }
   const double dorg = d[org];
   const double zl2 = rho * z[plo] * z[plo];
@@ -563,7 +574,17 @@ __device__ __forceinline__ void secular_root(const MergeDesc& md, const MergeBuf
     }
     if (G == 0.0) { a = b = t; conv = true; break; }
     if (G < 0.0) a = t; else b = t;
-    double tn = t - G / Gp;
+    // A Newton correction below the stopping size ends the iteration before
+    // the bracket test: t is an end of the bracket now, and t - dt can round
+    // onto it, which must not be taken for a step out of the bracket.
+    const double dt = G / Gp;
+    if (fabs(dt) <= 4.0 * DEPS * fabs(t)) {
+      const double tc = t - dt;
+      if (tc >= a && tc <= b) t = tc;
+      conv = true;
+      break;
+    }
+    double tn = t - dt;
     if (!(tn > a && tn < b)) tn = 0.5 * (a + b);
     const double step = fabs(tn - t);
     t = tn;
