@@ -243,8 +243,10 @@ int zq_tridiag_eig(zq_context* ctx, int n, const double* dd, const double* de, d
  * quaternion q = (a, b), i.e. the 2 x 2 complex block [[a, -conj b], [b,
  * conj a]], is stored as the 4 doubles (Re a, Im a, Re b, Im b); leading
  * dimensions count quaternions; lower_only != 0 writes only C[i][j] with
- * i >= j.  Device memory, asynchronous.  Returns 0, -2 / -3 bad trans, -4 a
- * negative size, -5 / -6 a null operand, -7 a leading dimension too small. */
+ * i >= j.  Device memory, asynchronous.  Tile rows follow the context's
+ * GEMM threshold for trailing size m (zq_set_tile_shape); they change no
+ * result.  Returns 0, -2 / -3 bad trans, -4 a negative size, -5 / -6 a null
+ * operand, -7 a leading dimension too small. */
 int zq_qgemm(zq_context* ctx, int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
              const double* B, int ldb, double beta, double* C, int ldc, int lower_only);
 

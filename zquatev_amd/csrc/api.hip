@@ -1412,6 +1412,8 @@ int zq_qgemm(zq_context* c, int transa, int transb, int m, int n, int k, double 
   const quat* qb = reinterpret_cast<const quat*>(B);
   const zq::QOperand oa{qa, qa, lda, lda, INT_MAX, ta, INT_MIN / 2};
   const zq::QOperand ob{qb, qb, ldb, ldb, INT_MAX, tb, INT_MIN / 2};
+  // tile rows by the context's GEMM threshold for trailing size m (zq_set_tile_shape)
+  zq::TileScope tiles(gemm64_min(c, m));
   // k == 0 runs no K stage: C = beta C
   return zq::qgemm(m, n, k, alpha, oa, ob, beta, reinterpret_cast<quat*>(C), ldc, lower_only != 0, c->stream);
 } catch (...) {
