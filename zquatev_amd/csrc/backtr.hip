@@ -96,53 +96,49 @@ __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__
 
 // T = (I + triu(G, 1))^{-1} for each block: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
 // T[j, j] = 1, zero below (the compact-WY recurrence with every tau = 1).
-// One workgroup per block (blockIdx.x), problem blockIdx.z.
-template <int WB>
+// G and T are stored per block of sub * WB reflectors (leading dimension
+// sub * WB); workgroup x factors the diagonal WB x WB sub-block x % sub of
+// stored block x / sub (problem blockIdx.z).  With sub = 2 the off-diagonal
+// block T12 = -T1 G12 T2 is formed afterwards by two GEMMs, and the second
+// workgroup of a block clears T21.
+constexpr int TF_WB = 64;
 __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__ Gall, quat* __restrict__ Tall,
-                                                 ZB zb) {
-  const int blk = blockIdx.x;
+                                                 int sub, ZB zb) {
+  constexpr int WB = TF_WB;
   if (zb.nvec) {
     n = zb.nvec[blockIdx.z];
     Gall = zoff(Gall, zb); Tall = zoff(Tall, zb);
   }
-  if (blk >= (n - 1 + WB - 1) / WB) return;
-  const int p = blk * WB;
-  const int wdt = min(WB, n - 1 - p);
-  const quat* G = Gall + (size_t)blk * WB * WB;
-  quat* T = Tall + (size_t)blk * WB * WB;
+  const int ldt = sub * WB;
+  const int sb = blockIdx.x / sub, h = blockIdx.x - sb * sub;
+  if (sb >= (n - 1 + ldt - 1) / ldt) return;
+  const int p = sb * ldt + h * WB;
+  const int wdt = min(WB, n - 1 - p);   // <= 0: no reflectors here, T is the identity
+  const quat* G = Gall + (size_t)sb * ldt * ldt + (size_t)h * WB * (ldt + 1);
+  quat* T = Tall + (size_t)sb * ldt * ldt + (size_t)h * WB * (ldt + 1);
   // Columns in groups of TB = 256 / WB, one per wave set: thread (r, c) (its
   // row r of T, column J + c of the group) first applies the columns l < J
-  // (T rows re-read from LDS or L2, the group's G entries staged in LDS);
-  // then the threads of c = 0 run the recurrence inside the group from
-  // those partials.  Every entry sums l = r .. jj-1 in increasing order, as
-  // the column-by-column recurrence does.  Blocks of 64 keep T itself in LDS
-  // (its rows are re-read for every later group) and write it out at the
-  // end; blocks of 128 do not fit and use global memory.
+  // (T rows re-read from LDS, the group's G entries staged in LDS); then the
+  // threads of c = 0 run the recurrence inside the group from those
+  // partials.  Every entry sums l = r .. jj-1 in increasing order, as the
+  // column-by-column recurrence does.  T stays in LDS (its rows are re-read
+  // for every later group) and is written out at the end.
   constexpr int TB = 256 / WB;
-  constexpr bool INL = WB <= 64;
   __shared__ quat gs[TB][WB];
   __shared__ quat ax[TB][WB];
-  __shared__ quat Ts[INL ? WB : 1][INL ? WB : 1];   // Ts[col][row]
-  auto tget = [&](int row, int col) -> quat {
-    if constexpr (INL) return Ts[col][row];
-    else return qload(T + row + (size_t)col * WB);
-  };
-  auto tset = [&](int row, int col, const quat& v) {
-    if constexpr (INL) Ts[col][row] = v;
-    else qstore(T + row + (size_t)col * WB, v);
-  };
+  __shared__ quat Ts[WB][WB];   // Ts[col][row]
   const int r = threadIdx.x % WB, cq = threadIdx.x / WB;
   for (int J = 0; J < WB; J += TB) {
     const int jq = J + cq;
     if (J >= wdt) {   // columns past the block's reflectors: identity (uniform branch)
-      tset(r, jq, r == jq ? quat{1.0, 0.0, 0.0, 0.0} : qzero());
+      Ts[jq][r] = r == jq ? quat{1.0, 0.0, 0.0, 0.0} : qzero();
       continue;
     }
-    gs[cq][r] = (jq < wdt && r < jq) ? qload(G + r + (size_t)jq * WB) : qzero();
+    gs[cq][r] = (jq < wdt && r < jq) ? qload(G + r + (size_t)jq * ldt) : qzero();
     __syncthreads();
     quat acc = qzero();
 #pragma unroll 4
-    for (int l = r; l < J; ++l) qfma(acc, tget(r, l), gs[cq][l]);
+    for (int l = r; l < J; ++l) qfma(acc, Ts[l][r], gs[cq][l]);
     ax[cq][r] = acc;
     __syncthreads();
     if (cq == 0) {
@@ -156,17 +152,16 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__
           if (r <= J + cc) qfma(a, tb[cc], gs[c][J + cc]);
         if (jj < wdt && r < jj) tb[c] = qscale(a, -1.0);
         else tb[c] = (r == jj) ? quat{1.0, 0.0, 0.0, 0.0} : qzero();
-        tset(r, jj, tb[c]);
+        Ts[jj][r] = tb[c];
       }
     }
     __syncthreads();
   }
-  if constexpr (INL) {
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < WB * WB; idx += 256) {
-      const int col = idx / WB, row = idx - col * WB;
-      qstore(T + row + (size_t)col * WB, Ts[col][row]);
-    }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < WB * WB; idx += 256) {
+    const int col = idx / WB, row = idx - col * WB;
+    qstore(T + row + (size_t)col * ldt, Ts[col][row]);
+    if (h == 1) qstore(T - (size_t)WB * ldt + row + (size_t)col * ldt, qzero());   // T21 of the stored block
   }
 }
 
@@ -180,8 +175,9 @@ size_t backtr_vt_slots(int n, int wbt) {
 
 size_t backtransform_batched_tasks(int nprob, const int* nh, int wbt, int nsplit) {
   if (nprob <= 0 || nh[0] < 2) return 1;
-  // per block: G and Y split products (<= nsplit each), VT and the X update
-  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * (2 * std::max(nsplit, 1) + 2) + 1;
+  // per block: G and Y split products (<= nsplit each), VT and the X update,
+  // and for blocks of two T sub-blocks the two products of T12
+  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * (2 * std::max(nsplit, 1) + 4) + 1;
 }
 
 size_t backtransform_batched_sums(int nprob, const int* nh, int wbt) {
@@ -243,6 +239,39 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
     }
   const int ngt = (int)tt.size() - gt0;
   const int ngs = (int)ss.size();
+  // blocks of two T sub-blocks: T12 = -T1 (G12 T2), P = G12 T2 in the
+  // (then free) split-K partial space, w1 / w2 reflectors in the halves
+  const int tsub = wbt / TF_WB;
+  const int pt0 = (int)tt.size();
+  if (tsub == 2)
+    for (int z = 0; z < nprob; ++z)
+      for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
+        const Blk k = block(z, b);
+        const int w1 = std::min(TF_WB, k.wdt), w2 = k.wdt - w1;
+        if (w2 <= 0) continue;
+        const quat* G = at(w.G, z) + (size_t)b * wbt * wbt;
+        const quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
+        quat* P = at(w.part, z) + (size_t)b * TF_WB * TF_WB;
+        QOperand G12{G + (size_t)TF_WB * wbt, G + (size_t)TF_WB * wbt, wbt, wbt, INT_MAX, false, INT_MIN / 2};
+        QOperand T2{T + TF_WB + (size_t)TF_WB * wbt, T + TF_WB + (size_t)TF_WB * wbt, wbt, wbt, INT_MAX, false,
+                    INT_MIN / 2};
+        tt.push_back(QGemmTask{w1, w2, w2, 0, w2, TF_WB, 0, 1.0, 0.0, G12, T2, P});
+      }
+  const int npt = (int)tt.size() - pt0;
+  const int t120 = (int)tt.size();
+  if (tsub == 2)
+    for (int z = 0; z < nprob; ++z)
+      for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
+        const Blk k = block(z, b);
+        const int w1 = std::min(TF_WB, k.wdt), w2 = k.wdt - w1;
+        if (w2 <= 0) continue;
+        quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
+        const quat* P = at(w.part, z) + (size_t)b * TF_WB * TF_WB;
+        QOperand T1{T, T, wbt, wbt, INT_MAX, false, INT_MIN / 2};
+        QOperand Po{P, P, TF_WB, TF_WB, INT_MAX, false, INT_MIN / 2};
+        tt.push_back(QGemmTask{w1, w2, w1, 0, w1, wbt, 0, -1.0, 0.0, T1, Po, T + (size_t)TF_WB * wbt});
+      }
+  const int nt12 = (int)tt.size() - t120;
   const int vt0 = (int)tt.size();
   int vtmaxm = 0;
   for (int z = 0; z < nprob; ++z)
@@ -309,9 +338,10 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
   // phase A launches
   if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, false, s))) return rc;
   hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(wbt * wbt, 256), 1, ngs), dim3(256), 0, s, sums);
-  if (wbt == 64) hipLaunchKernelGGL(k_tfactor<64>, dim3(nblk, 1, nprob), dim3(256), 0, s, L, w.G, w.T, zb);
-  else if (wbt == 128) hipLaunchKernelGGL(k_tfactor<128>, dim3(nblk, 1, nprob), dim3(256), 0, s, L, w.G, w.T, zb);
-  else return ZQI_ERR_ARG;
+  if (wbt != TF_WB && wbt != 2 * TF_WB) return ZQI_ERR_ARG;
+  hipLaunchKernelGGL(k_tfactor, dim3(nblk * tsub, 1, nprob), dim3(256), 0, s, L, w.G, w.T, tsub, zb);
+  if ((rc = qgemm_tasks(tasks + pt0, npt, TF_WB, TF_WB, false, false, false, s))) return rc;
+  if ((rc = qgemm_tasks(tasks + t120, nt12, TF_WB, TF_WB, false, false, false, s))) return rc;
   if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, false, s))) return rc;
   // phase B launches
   for (const Step& q : steps) {
