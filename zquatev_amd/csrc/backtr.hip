@@ -97,10 +97,12 @@ __global__ void __launch_bounds__(256) k_sum_parts_b(const SumTask* __restrict__
 // T = (I + triu(G, 1))^{-1} for each block: T[0:j, j] = -T[0:j, 0:j] G[0:j, j],
 // T[j, j] = 1, zero below (the compact-WY recurrence with every tau = 1).
 // G and T are stored per block of sub * WB reflectors (leading dimension
-// sub * WB); workgroup x factors the diagonal WB x WB sub-block x % sub of
-// stored block x / sub (problem blockIdx.z).  With sub = 2 the off-diagonal
-// block T12 = -T1 G12 T2 is formed afterwards by two GEMMs, and the second
-// workgroup of a block clears T21.
+// sub * WB); workgroup x factors the diagonal WB x WB sub-block h = x % sub
+// of stored block x / sub (problem blockIdx.z) and clears the part of T left
+// of it (rows h WB .. + WB, columns 0 .. h WB).  For sub > 1 the blocks above
+// the diagonal are formed afterwards by GEMMs, pairs of neighbouring
+// sub-blocks of width w = WB, 2 WB, ... merged level by level:
+// T12 = -T1 (G12 T2).
 constexpr int TF_WB = 64;
 __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__ Gall, quat* __restrict__ Tall,
                                                  int sub, ZB zb) {
@@ -161,7 +163,10 @@ __global__ void __launch_bounds__(256) k_tfactor(int n, const quat* __restrict__
   for (int idx = threadIdx.x; idx < WB * WB; idx += 256) {
     const int col = idx / WB, row = idx - col * WB;
     qstore(T + row + (size_t)col * ldt, Ts[col][row]);
-    if (h == 1) qstore(T - (size_t)WB * ldt + row + (size_t)col * ldt, qzero());   // T21 of the stored block
+  }
+  for (int idx = threadIdx.x; idx < WB * h * WB; idx += 256) {   // below the diagonal, left of this sub-block
+    const int col = idx / WB, row = idx - col * WB;
+    qstore(T - (size_t)h * WB * ldt + row + (size_t)col * ldt, qzero());
   }
 }
 
@@ -176,8 +181,8 @@ size_t backtr_vt_slots(int n, int wbt) {
 size_t backtransform_batched_tasks(int nprob, const int* nh, int wbt, int nsplit) {
   if (nprob <= 0 || nh[0] < 2) return 1;
   // per block: G and Y split products (<= nsplit each), VT and the X update,
-  // and for blocks of two T sub-blocks the two products of T12
-  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * (2 * std::max(nsplit, 1) + 4) + 1;
+  // and two products per merge of T sub-blocks (wbt / 64 - 1 merges)
+  return (size_t)nprob * cdiv(nh[0] - 1, wbt) * (2 * std::max(nsplit, 1) + 2 * (wbt / 64) + 2) + 1;
 }
 
 size_t backtransform_batched_sums(int nprob, const int* nh, int wbt) {
@@ -239,39 +244,46 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
     }
   const int ngt = (int)tt.size() - gt0;
   const int ngs = (int)ss.size();
-  // blocks of two T sub-blocks: T12 = -T1 (G12 T2), P = G12 T2 in the
-  // (then free) split-K partial space, w1 / w2 reflectors in the halves
+  // T sub-blocks merged level by level: at width w, the pair (q, q + w) of
+  // each block gets T12 = -T1 (G12 T2), P = G12 T2 in the (then free)
+  // split-K partial space; w1 / w2 reflectors in the two halves
   const int tsub = wbt / TF_WB;
-  const int pt0 = (int)tt.size();
-  if (tsub == 2)
-    for (int z = 0; z < nprob; ++z)
-      for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
-        const Blk k = block(z, b);
-        const int w1 = std::min(TF_WB, k.wdt), w2 = k.wdt - w1;
-        if (w2 <= 0) continue;
-        const quat* G = at(w.G, z) + (size_t)b * wbt * wbt;
-        const quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
-        quat* P = at(w.part, z) + (size_t)b * TF_WB * TF_WB;
-        QOperand G12{G + (size_t)TF_WB * wbt, G + (size_t)TF_WB * wbt, wbt, wbt, INT_MAX, false, INT_MIN / 2};
-        QOperand T2{T + TF_WB + (size_t)TF_WB * wbt, T + TF_WB + (size_t)TF_WB * wbt, wbt, wbt, INT_MAX, false,
-                    INT_MIN / 2};
-        tt.push_back(QGemmTask{w1, w2, w2, 0, w2, TF_WB, 0, 1.0, 0.0, G12, T2, P});
-      }
-  const int npt = (int)tt.size() - pt0;
-  const int t120 = (int)tt.size();
-  if (tsub == 2)
-    for (int z = 0; z < nprob; ++z)
-      for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
-        const Blk k = block(z, b);
-        const int w1 = std::min(TF_WB, k.wdt), w2 = k.wdt - w1;
-        if (w2 <= 0) continue;
-        quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
-        const quat* P = at(w.part, z) + (size_t)b * TF_WB * TF_WB;
-        QOperand T1{T, T, wbt, wbt, INT_MAX, false, INT_MIN / 2};
-        QOperand Po{P, P, TF_WB, TF_WB, INT_MAX, false, INT_MIN / 2};
-        tt.push_back(QGemmTask{w1, w2, w1, 0, w1, wbt, 0, -1.0, 0.0, T1, Po, T + (size_t)TF_WB * wbt});
-      }
-  const int nt12 = (int)tt.size() - t120;
+  struct Level { int p0, np, t0, nt; };
+  std::vector<Level> levels;
+  for (int wl = TF_WB; wl < wbt; wl *= 2) {
+    Level lv{};
+    auto pairs = [&](bool second) {
+      for (int z = 0; z < nprob; ++z)
+        for (int b = 0; b < nblk && b * wbt < nh[z] - 1; ++b) {
+          const Blk k = block(z, b);
+          for (int q = 0; q + wl < wbt; q += 2 * wl) {
+            const int w1 = std::min(wl, std::max(0, k.wdt - q)), w2 = std::min(wl, std::max(0, k.wdt - q - wl));
+            if (w2 <= 0) continue;
+            quat* T = at(w.T, z) + (size_t)b * wbt * wbt;
+            quat* P = at(w.part, z) + (size_t)b * wbt * wbt + (size_t)(q / (2 * wl)) * wl * wl;
+            if (!second) {
+              const quat* G12 = at(w.G, z) + (size_t)b * wbt * wbt + q + (size_t)(q + wl) * wbt;
+              const quat* T2 = T + (q + wl) + (size_t)(q + wl) * wbt;
+              QOperand Ga{G12, G12, wbt, wbt, INT_MAX, false, INT_MIN / 2};
+              QOperand Tb{T2, T2, wbt, wbt, INT_MAX, false, INT_MIN / 2};
+              tt.push_back(QGemmTask{w1, w2, w2, 0, w2, wl, 0, 1.0, 0.0, Ga, Tb, P});
+            } else {
+              const quat* T1 = T + q + (size_t)q * wbt;
+              QOperand Ta{T1, T1, wbt, wbt, INT_MAX, false, INT_MIN / 2};
+              QOperand Pb{P, P, wl, wl, INT_MAX, false, INT_MIN / 2};
+              tt.push_back(QGemmTask{w1, w2, w1, 0, w1, wbt, 0, -1.0, 0.0, Ta, Pb, T + q + (size_t)(q + wl) * wbt});
+            }
+          }
+        }
+    };
+    lv.p0 = (int)tt.size();
+    pairs(false);
+    lv.np = (int)tt.size() - lv.p0;
+    lv.t0 = (int)tt.size();
+    pairs(true);
+    lv.nt = (int)tt.size() - lv.t0;
+    levels.push_back(lv);
+  }
   const int vt0 = (int)tt.size();
   int vtmaxm = 0;
   for (int z = 0; z < nprob; ++z)
@@ -338,10 +350,13 @@ int backtransform_batched(int nprob, const int* nh, ZB zb, const int* zrow_d, co
   // phase A launches
   if ((rc = qgemm_tasks(tasks + gt0, ngt, gmaxk, gmaxk, true, false, false, s))) return rc;
   hipLaunchKernelGGL(k_sum_parts_b, dim3(cdiv(wbt * wbt, 256), 1, ngs), dim3(256), 0, s, sums);
-  if (wbt != TF_WB && wbt != 2 * TF_WB) return ZQI_ERR_ARG;
+  if (wbt < TF_WB || wbt % TF_WB || (tsub & (tsub - 1))) return ZQI_ERR_ARG;
   hipLaunchKernelGGL(k_tfactor, dim3(nblk * tsub, 1, nprob), dim3(256), 0, s, L, w.G, w.T, tsub, zb);
-  if ((rc = qgemm_tasks(tasks + pt0, npt, TF_WB, TF_WB, false, false, false, s))) return rc;
-  if ((rc = qgemm_tasks(tasks + t120, nt12, TF_WB, TF_WB, false, false, false, s))) return rc;
+  for (size_t li = 0; li < levels.size(); ++li) {
+    const int wl = TF_WB << li;
+    if ((rc = qgemm_tasks(tasks + levels[li].p0, levels[li].np, wl, wl, false, false, false, s))) return rc;
+    if ((rc = qgemm_tasks(tasks + levels[li].t0, levels[li].nt, wl, wl, false, false, false, s))) return rc;
+  }
   if ((rc = qgemm_tasks(tasks + vt0, nvt, vtmaxm, wbt, false, true, false, s))) return rc;
   // phase B launches
   for (const Step& q : steps) {
