@@ -213,7 +213,7 @@ struct CaptureScope {
 // Reflectors per compact-WY block of the back-transform.  Batches group
 // their problems by this width (as by panel width), so that a member applies
 // the same blocks as its solo solve.
-inline int backtr_width(int n) { return n > 384 ? 128 : 64; }
+inline int backtr_width(int n) { return n > 448 ? 128 : 64; }
 
 struct Plan {
   quat* M;
