@@ -973,11 +973,12 @@ const char* zq_error_string(int code) {
 }
 
 int zq_tuned_panel_width(int n) {
-  // measured on MI355X: nb = 16 is 1-3 % faster than 32 for single solves of
-  // n = 128 .. 1024 (its column kernel holds half the panel coefficients per
+  // measured on MI355X: nb = 16 is 0.3-3 % faster than 32 for single solves of
+  // n = 128 .. 768 (its column kernel holds half the panel coefficients per
   // thread, so more blocks fit a CU) and 5 % for a batch of 512 problems of
-  // n <= 512; 32 is fastest above.  zq_autotune refines it per size bucket.
-  return n <= 1024 ? 16 : 32;
+  // n <= 512; 32 is fastest from n = 1024 (0.9 % there).  zq_autotune refines
+  // it per size bucket (floor(log2 n)).
+  return n < 1024 ? 16 : 32;
 }
 
 int zq_create(zq_context** out, int device) try {
