@@ -57,9 +57,12 @@ def parse():
     ap.add_argument("--no-values-only", action="store_true", help="skip the eigenvalues-only (jobz='N') summary")
     ap.add_argument("--no-host-entry", action="store_true",
                     help="skip the host-memory (drop-in, PCIe-inclusive) summary")
-    ap.add_argument("--workload", choices=("single", "batched", "serving"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "serving", "qgemm"), default="single",
                     help="single: the 2n=8192 solve (plus a batched summary); batched: the batched workload only; "
-                         "serving: the Poisson request stream through the server and the plain loop")
+                         "serving: the Poisson request stream through the server and the plain loop; qgemm: the "
+                         "quaternion GEMM alone (zq_qgemm, C -= A B), for its MFMA roofline and counters")
+    ap.add_argument("--qgemm-shape", default="4096,4096,1024", metavar="M,N,K",
+                    help="qgemm: quaternion sizes of C (M x N) and the inner dimension K")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--loop-seconds", type=float, default=10.0, help="arrival window replayed through the plain loop")
     ap.add_argument("--rate", type=float, default=200.0)
@@ -298,6 +301,45 @@ def run_batched(ctx, stream, count, seed, steps, warmup, check=True, profile=Fal
     return out
 
 
+def run_qgemm(args):
+    """The quaternion GEMM of the trailing update and back-transform on its
+    own: C (M x N) -= A (M x K) B (K x N), FP64 MFMA, random operands resident
+    in HBM, the library's stream; warm-up and timed calls back to back."""
+    import torch
+    import zquatev_amd as zq
+    m, n, k = (int(v) for v in args.qgemm_shape.split(","))
+    dev = torch.device("cuda:0")
+    ctx = zq.Context(0)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    if args.tile64 > 0:
+        ctx.set_tile_shape(-1, args.tile64)
+    g = torch.Generator(device=dev).manual_seed(11)
+    A = torch.randn(4 * m * k, dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn(4 * k * n, dtype=torch.float64, device=dev, generator=g)
+    C = torch.zeros(4 * m * n, dtype=torch.float64, device=dev)
+
+    def call():
+        rc = ctx.qgemm_device("N", "N", m, n, k, -1.0, A.data_ptr(), m, B.data_ptr(), k, 1.0, C.data_ptr(), m)
+        if rc != 0:
+            raise RuntimeError(f"zq_qgemm failed ({rc})")
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    tf = 32.0 * m * n * k / t / 1e12
+    print(json.dumps({"metric": "quaternion GEMM TFLOP/s", "value": round(tf, 2), "unit": "TFLOP/s",
+                      "ms_per_call": round(1000 * t, 3), "frac_of_fp64_mfma_peak": round(tf / FP64_MFMA_PEAK_TFS, 4),
+                      "dtype": "f64", "config": {"workload": f"zq_qgemm C({m}x{n}) -= A({m}x{k}) B({k}x{n})",
+                                                 "flops_per_call": 32.0 * m * n * k}}))
+    return 0
+
+
 def run_serving(args, summary=False):
     """The serving workload: Poisson arrivals (rate args.rate) for
     args.serve_seconds; 2n log-uniform over even [128, 4096]; 70 % of the
@@ -526,6 +568,8 @@ def main():
 
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "qgemm":
+        return run_qgemm(args)
     if args.workload == "batched":
         ctx = zq.Context(0)
         stream = torch.cuda.current_stream(torch.device("cuda:0"))
