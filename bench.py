@@ -756,6 +756,23 @@ def main():
                 "kernel": "k_qgemm (%s)" % ("trailing rank-2nb update" if dom == "trail" else "back-transform")}
     roof["avg_launch_ms"] = kd["avg_ms"]
     roof["launches"] = kd["launches"]
+    if dom == "symv":
+        # the streaming read rate this device reaches (torch's sum over 4 GiB
+        # resident in HBM, best of 5), beside the 8 TB/s datasheet peak
+        xs = torch.ones(1 << 29, dtype=torch.float64, device=dev)
+        xs.sum()
+        best = None
+        for _ in range(5):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            xs.sum()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            best = dt if best is None else min(best, dt)
+        rd = xs.numel() * 8 / best / 1e9
+        roof["measured_stream_read_GBps"] = round(rd, 1)
+        roof["frac_of_measured_stream_read"] = round(roof["achieved"] / rd, 4)
+        del xs
     for c in kernels:
         v = kernels[c]
         if v["launches"] and v["units"]:

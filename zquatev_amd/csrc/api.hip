@@ -325,6 +325,10 @@ void stedc_layout(Bump& b, int ntot, int nprob, int lq, bool vectors, zq::StedcW
     s.posD = b.take<int>(nt);
     s.nrot = b.take<int>(nt);
     s.kcnt = b.take<int>(nt);
+    s.mix = b.take<int>(nt);
+    s.gpos = b.take<int>(nt);
+    s.ginv = b.take<int>(nt);
+    s.kgrp = b.take<int>(nt);
   } else {
     s.Z = s.Qtmp = s.U = nullptr;
   }

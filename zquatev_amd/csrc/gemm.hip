@@ -339,7 +339,11 @@ int splitk_count(int K, int nsplit, int* kchunk_out) {
 // per blockIdx.z: for the merge of rows [lo, hi) (ns = hi - lo) of problem
 // base ob,  Z[lo:hi, lo - ob + posK[j]] = Qtmp_m (ns x K) U_m (K x K), with
 // K = kcnt[z] read on the device (data-dependent after deflation) and the
-// merge's Qtmp / U blocks at lo * lq (ld ns).
+// merge's Qtmp / U blocks at lo * lq (ld ns).  Qtmp's columns come in three
+// groups: [0, g0) hold rows of the first half (rows < mid - lo) only, [g0,
+// g1) both halves, [g1, K) the second half only (kgrp[2z], kgrp[2z + 1]), so
+// a tile of first-half rows sums over [0, g1) and one of second-half rows
+// over [g0, K), skipping exact zeros (as LAPACK's dlaed3 splits the product).
 // ---------------------------------------------------------------------------
 constexpr int DBM = 64, DBN = 64, DBK = 16;
 
@@ -347,14 +351,17 @@ constexpr int DBM = 64, DBN = 64, DBK = 16;
 // through double-buffered LDS; the next stage's operands are fetched into
 // registers before the current stage's MFMAs (one barrier per stage).
 __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict__ descs, const int* __restrict__ kcnt,
-                                                     const double* __restrict__ Qtmp, const double* __restrict__ U,
-                                                     int lq, double* __restrict__ Z, int ldz,
-                                                     const int* __restrict__ posK) {
+                                                     const int* __restrict__ kgrp, const double* __restrict__ Qtmp,
+                                                     const double* __restrict__ U, int lq, double* __restrict__ Z,
+                                                     int ldz, const int* __restrict__ posK) {
   const MergeDesc md = descs[blockIdx.z];
   const int K = kcnt[blockIdx.z];
-  const int m = md.hi - md.lo;
+  const int m = md.hi - md.lo, n1 = md.mid - md.lo;
   const int row0 = blockIdx.x * DBM, col0 = blockIdx.y * DBN;
   if (col0 >= K || row0 >= m) return;
+  // the sum's range for this tile's rows (the full range for a tile across the halves)
+  const int kb = row0 >= n1 ? kgrp[2 * blockIdx.z] : 0;
+  const int ke = row0 + DBM <= n1 ? kgrp[2 * blockIdx.z + 1] : K;
   const double* A = Qtmp + (size_t)md.lo * lq;
   const double* B = U + (size_t)md.lo * lq;
   const int lda = m, ldb = m;
@@ -372,10 +379,10 @@ __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict
       const int idx = tid + 256 * h;
       const int i = idx & (DBM - 1), k = idx >> 6;
       const int gi = row0 + i, gk = k0 + k;
-      ra[h] = (gi < m && gk < K) ? A[gi + (size_t)gk * lda] : 0.0;
-      const int kb = idx & (DBK - 1), jb = idx / DBK;
-      const int gkb = k0 + kb, gj = col0 + jb;
-      rb[h] = (gkb < K && gj < K) ? B[gkb + (size_t)gj * ldb] : 0.0;
+      ra[h] = (gi < m && gk < ke) ? A[gi + (size_t)gk * lda] : 0.0;
+      const int kr = idx & (DBK - 1), jb = idx / DBK;
+      const int gkb = k0 + kr, gj = col0 + jb;
+      rb[h] = (gkb < ke && gj < K) ? B[gkb + (size_t)gj * ldb] : 0.0;
     }
   };
   auto stash = [&](int buf) {
@@ -391,12 +398,12 @@ __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  fetch(0);
+  fetch(kb);
   stash(0);
   __syncthreads();
   int cur = 0;
-  for (int k0 = 0; k0 < K; k0 += DBK) {
-    const bool more = k0 + DBK < K;
+  for (int k0 = kb; k0 < ke; k0 += DBK) {
+    const bool more = k0 + DBK < ke;
     if (more) fetch(k0 + DBK);
 #pragma unroll
     for (int kk = 0; kk < DBK; kk += 4) {
@@ -429,11 +436,11 @@ __global__ void __launch_bounds__(256) k_dgemm_merge(const MergeDesc* __restrict
     }
 }
 
-int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const double* Qtmp,
-                 const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s) {
+int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const int* kgrp,
+                 const double* Qtmp, const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s) {
   if (nmerge <= 0 || nsmax <= 0) return ZQI_OK;
   dim3 grid((nsmax + DBM - 1) / DBM, (nsmax + DBN - 1) / DBN, nmerge);
-  launch_k(k_dgemm_merge, grid, dim3(256), 0, s, descs_dev, kcnt, Qtmp, U, lq, Z, ldz, posK);
+  launch_k(k_dgemm_merge, grid, dim3(256), 0, s, descs_dev, kcnt, kgrp, Qtmp, U, lq, Z, ldz, posK);
   return hipGetLastError() == hipSuccess ? ZQI_OK : ZQI_ERR_HIP;
 }
 

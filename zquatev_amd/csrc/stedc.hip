@@ -226,6 +226,15 @@ struct MergeBufs {
   int lq;        // the merge of rows [lo, hi) keeps its Qtmp and U blocks at lo * lq
   int* info;     // status word per problem (non-convergence of a secular root)
   int cap;       // secular-equation steps per root
+  // The merge product Z = Qtmp U skips the zero blocks of Qtmp: a kept
+  // column holds the first half's rows only, the second half's only, or (after
+  // a deflation rotation across the halves) both.  Qtmp's columns and U's
+  // rows are in GEMM order: first-half columns, both-halves, second-half,
+  // each group in kept order.
+  int* mix;      // per sorted position: the column mixes both halves
+  int* gpos;     // per kept index: its GEMM position
+  int* ginv;     // per GEMM position: the kept index
+  int* kgrp;     // per merge: end of the first-half group, end of the both-halves group
 };
 
 __device__ __forceinline__ int count_less(const double* a, int len, double v) {
@@ -259,8 +268,9 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
   const double beta = es[mid - 1];
   const double rho = 2.0 * beta;
   __shared__ double s_dk[PREP_LDS], s_zk[PREP_LDS];
-  __shared__ int s_keep[PREP_LDS], s_dfl[PREP_LDS];
+  __shared__ int s_keep[PREP_LDS], s_dfl[PREP_LDS], s_mix[PREP_LDS];
   const bool inl = ns <= PREP_LDS;   // uniform
+  int* mix = inl ? s_mix : B.mix + lo;
   double* dk = inl ? s_dk : B.dk + lo;
   double* zk = inl ? s_zk : B.zk + lo;
   int* src = B.src + lo;
@@ -279,6 +289,7 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
     dk[r] = v;
     zk[r] = zr * 0.70710678118654752440;
     src[r] = t;
+    mix[r] = 0;
   }
   __syncthreads();
   __shared__ double shd[4], shz[4];
@@ -370,6 +381,8 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
       if (fabs(t * c * s) <= tol) {
         zk[j] = tt;
         zk[pj] = 0.0;
+        // column j now holds a combination of columns pj and j of Z
+        mix[j] |= mix[pj] | ((src[pj] < n1) != (src[j] < n1));
         rot[4 * nr + 0] = (double)pj;
         rot[4 * nr + 1] = (double)j;
         rot[4 * nr + 2] = c;
@@ -404,6 +417,49 @@ __global__ void __launch_bounds__(256) k_merge_prep(const MergeDesc* __restrict_
       B.dval[lo + t - K] = dk[dfl[t - K]];
       if (inl) B.dfl[lo + t - K] = dfl[t - K];
     }
+    if (inl) B.mix[lo + t] = mix[t];
+  }
+}
+
+// GEMM order of the kept columns (one workgroup per merge): first-half
+// columns, then the ones mixing both halves, then second-half columns, each
+// group in kept order (stable ranks from block scans over contiguous chunks).
+__global__ void __launch_bounds__(256) k_merge_types(const MergeDesc* __restrict__ descs, MergeBufs B) {
+  const MergeDesc md = descs[blockIdx.x];
+  const int lo = md.lo, n1 = md.mid - md.lo;
+  const int K = B.kcnt[blockIdx.x];
+  const int* keep = B.keep + lo;
+  auto type_of = [&](int k) {
+    const int sp = keep[k];
+    return B.mix[lo + sp] ? 1 : (B.src[lo + sp] < n1 ? 0 : 2);
+  };
+  const int C = (K + 255) / 256;
+  const int k0 = min(K, (int)threadIdx.x * C), k1 = min(K, k0 + C);
+  int c[3] = {0, 0, 0};
+  for (int k = k0; k < k1; ++k) ++c[type_of(k)];
+  __shared__ int sc[3][256];
+#pragma unroll
+  for (int g = 0; g < 3; ++g) sc[g][threadIdx.x] = c[g];
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {   // inclusive scans of the three counts
+    int v[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) v[g] = threadIdx.x >= off ? sc[g][threadIdx.x - off] : 0;
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < 3; ++g) sc[g][threadIdx.x] += v[g];
+    __syncthreads();
+  }
+  const int n0 = sc[0][255], nm = sc[1][255];
+  int pos[3] = {sc[0][threadIdx.x] - c[0], n0 + sc[1][threadIdx.x] - c[1], n0 + nm + sc[2][threadIdx.x] - c[2]};
+  for (int k = k0; k < k1; ++k) {
+    const int p = pos[type_of(k)]++;
+    B.gpos[lo + k] = p;
+    B.ginv[lo + p] = k;
+  }
+  if (threadIdx.x == 0) {
+    B.kgrp[2 * blockIdx.x] = n0;
+    B.kgrp[2 * blockIdx.x + 1] = n0 + nm;
   }
 }
 
@@ -429,8 +485,8 @@ __global__ void __launch_bounds__(256) k_merge_rot(const MergeDesc* __restrict__
   }
 }
 
-// Qtmp = the merge's block of Z with its columns in the order kept, then
-// deflated.  Grid: x = 256-row chunks, y = GATHER_COLS-column chunks, z = merge.
+// Qtmp = the merge's block of Z with its columns in the order kept (in GEMM
+// order), then deflated.  Grid: x = 256-row chunks, y = GATHER_COLS-column chunks, z = merge.
 constexpr int GATHER_COLS = 8;
 __global__ void __launch_bounds__(256) k_merge_gather(const MergeDesc* __restrict__ descs,
                                                       const double* __restrict__ Z, int ldz, MergeBufs B) {
@@ -447,7 +503,7 @@ __global__ void __launch_bounds__(256) k_merge_gather(const MergeDesc* __restric
 #pragma unroll
   for (int u = 0; u < GATHER_COLS; ++u) {
     const int k = min(k0 + u, ns - 1);
-    const int sp = (k < K) ? B.keep[lo + k] : B.dfl[lo + k - K];
+    const int sp = (k < K) ? B.keep[lo + B.ginv[lo + k]] : B.dfl[lo + k - K];
     col[u] = cb + src[sp];
   }
   double v[GATHER_COLS];
@@ -666,10 +722,11 @@ __global__ void __launch_bounds__(256) k_merge_buildU(const MergeDesc* __restric
   const double t = B.tau[lo + i];
   const double dorg = d[o];
   double* Ucol = B.U + (size_t)lo * B.lq + (size_t)i * ns;
+  const int* gpos = B.gpos + lo;
   double ss = 0.0;
-  for (int j = threadIdx.x; j < K; j += 256) {
+  for (int j = threadIdx.x; j < K; j += 256) {   // row j of the kept order at its GEMM position
     const double v = zh[j] / ((d[j] - dorg) - t);
-    Ucol[j] = v;
+    Ucol[gpos[j]] = v;
     ss += v * v;
   }
   __shared__ double sh[4];
@@ -943,7 +1000,8 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
                        sw.es, sw.lam);
   hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf), dim3(64), 0, s, ileaf, sw.lam, sw.es, sw.Z, ldz, sw.info, sw.cap_leaf);
   MergeBufs B{sw.dk, sw.zk, sw.src, sw.keep, sw.dfl, sw.rot, sw.nrot, sw.kcnt, sw.dkk, sw.zkk, sw.dval, sw.rho,
-              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U, lq, sw.info, sw.cap_secular};
+              sw.org, sw.tau, sw.zhat, sw.posK, sw.posD, sw.Qtmp, sw.U, lq, sw.info, sw.cap_secular,
+              sw.mix, sw.gpos, sw.ginv, sw.kgrp};
   for (size_t li = 0; li < level_off.size(); ++li) {
     const size_t off = level_off[li];
     const size_t cnt = (li + 1 < level_off.size() ? level_off[li + 1] : flat.size()) - off;
@@ -953,6 +1011,7 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     const int nm = (int)cnt;
     hipLaunchKernelGGL(k_merge_prep, dim3(nm), dim3(256), 0, s, dd, sw.lam, sw.es, sw.Z, ldz, B);
     hipLaunchKernelGGL(k_merge_rot, dim3(cdiv(nsmax, 256), nm), dim3(256), 0, s, dd, sw.Z, ldz, B);
+    hipLaunchKernelGGL(k_merge_types, dim3(nm), dim3(256), 0, s, dd, B);
     hipLaunchKernelGGL(k_merge_gather, dim3(cdiv(nsmax, 256), cdiv(nsmax, GATHER_COLS), nm), dim3(256), 0, s, dd,
                        sw.Z, ldz, B);
     {
@@ -968,7 +1027,7 @@ int stedc(int nprob, const int* offs, const int* ns, int ntot, int lq, const dou
     hipLaunchKernelGGL(k_merge_order, dim3(nm), dim3(256), 0, s, dd, sw.lam, B);
     {
       ProfScope ps(PROF_MERGE, s, 0.0);
-      rc = dgemm_merges(dd, nm, nsmax, sw.kcnt, sw.Qtmp, sw.U, lq, sw.Z, ldz, sw.posK, s);
+      rc = dgemm_merges(dd, nm, nsmax, sw.kcnt, sw.kgrp, sw.Qtmp, sw.U, lq, sw.Z, ldz, sw.posK, s);
     }
     if (rc) return rc;
     hipLaunchKernelGGL(k_merge_copydefl, dim3(cdiv(nsmax, 256), nsmax, nm), dim3(256), 0, s, dd, sw.Z, ldz, B);

@@ -338,6 +338,10 @@ struct StedcWork {
   double* scal;   // 16
   int *src, *keep, *dfl, *org, *posK, *posD;                             // n each
   int *nrot, *kcnt;                                                      // n each
+  // merge GEMM structure: mixed flag per sorted position, GEMM position of
+  // each kept column and its inverse, and per merge the ends of the groups
+  // of kept columns from the first half and from both halves (2 per merge)
+  int *mix, *gpos, *ginv, *kgrp;                                         // n each
   int* info;      // one status word per problem (ZQI_INFO_NONFINITE | non-convergence: failing local row + 1)
   int* ints;      // tree descriptors
   size_t int_slots;
@@ -492,9 +496,11 @@ struct TileScope {
 };
 // number of K chunks (and the chunk length) a split-K product over K uses
 int splitk_count(int K, int nsplit, int* kchunk_out);
-// real merge products of one divide & conquer tree level (see stedc.hip)
-int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const double* Qtmp,
-                 const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s);
+// real merge products of one divide & conquer tree level (see stedc.hip);
+// kgrp: per merge the GEMM positions ending the first-half and both-halves
+// groups of kept columns
+int dgemm_merges(const MergeDesc* descs_dev, int nmerge, int nsmax, const int* kcnt, const int* kgrp,
+                 const double* Qtmp, const double* U, int lq, double* Z, int ldz, const int* posK, hipStream_t s);
 int mfma_probe(const double* a, const double* b, double* out, hipStream_t s);
 
 // api.hip hooks for the serving loop (serve.hip)
