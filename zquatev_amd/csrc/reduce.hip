@@ -57,24 +57,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Tail-norm partials of the first column c of a panel (no update is pending
-// there): part[b] = sum of |M[i, c]|^2 over rows i >= c + 2 of the row block
-// [c + RQ b, c + RQ (b + 1)) -- the row blocks in which k_wcol leaves them for
-// every later column of the panel.  Each 256-thread block covers 16 row blocks.
-__global__ void __launch_bounds__(256) k_col_norm(const quat* __restrict__ M, int ld, int n, int c,
-                                                  double* __restrict__ part, ZB zb) {
-  if (zb.nvec) {
-    n = zb.nvec[blockIdx.z];
-    M = zoff(M, zb); part = zoff(part, zb);
-  }
-  const int i = c + blockIdx.x * 256 + threadIdx.x;
-  double v = (i >= c + 2 && i < n) ? qnorm2(qload(M + i + (size_t)c * ld)) : 0.0;
-#pragma unroll
-  for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
-  const int pb = blockIdx.x * 16 + (threadIdx.x >> 4);
-  if ((threadIdx.x & 15) == 0 && c + pb * RQ < n) part[pb] = v;
-}
-
 // Reflector for column c: x = M[c+1:, c].  H x = alpha e_1 with
 // alpha = -q |x|, q = x_1/|x_1|; u = gamma (x - alpha e_1), gamma = 1/sqrt(|x| (|x| + |x_1|)),
 // so u^* u = 2.  A zero tail gives u = 0 and alpha = x_1.  Computed identically
@@ -120,7 +102,8 @@ __device__ __forceinline__ Refl reflector(const quat& x1, double tail, quat* alp
 // every (slot, row) written exactly once, so the sum is deterministic.
 // Blocks are enumerated row-group major: group g has SPG (g + 1) strips.
 // The max(1, ceil(8 j / 64)) blocks before the ntiles(G) tiles compute the
-// reflector from the tail-norm partials (the first of them publishes gamma,
+// reflector from the tail-norm partials (from the column itself for a
+// panel's first column, j = 0) (the first of them publishes gamma,
 // u1, x1 in scal and d[c], e[c]) and finish the tall-skinny products
 // TS = [W^* u | V^* u]: they add the per-block partials of W^* x and V^* x
 // that k_wcol left in TSP (the waves over the partial index, then the
@@ -191,7 +174,11 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     }
     const quat x1 = qload(M + s0 + (size_t)c * ld);
     double tail = 0.0;
-    for (int k = tid; k < nparts; k += SYT) tail += part[k];
+    if (j == 0) {   // a panel's first column has no pending update: its tail norm from the column itself
+      for (int i = c + 2 + tid; i < n; i += SYT) tail += qnorm2(qload(M + i + (size_t)c * ld));
+    } else {        // else from the partials the column kernel left for it
+      for (int k = tid; k < nparts; k += SYT) tail += part[k];
+    }
     tail = block_sum(tail, sh);
     quat alpha;
     const Refl R = reflector(x1, tail, &alpha);
@@ -685,9 +672,9 @@ static void launch_wcol(int kpt, int sz, dim3 grid, hipStream_t s, Args... args)
   }
 }
 
-// Per panel: k_col_norm for its first column, then per column k_symv and
-// k_wcol (which also updates the next column of the panel); then the
-// trailing update.
+// Per panel: per column k_symv and k_wcol (which also updates the next
+// column of the panel and leaves its tail-norm partials); then the trailing
+// update.
 int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork& w, hipStream_t s) {
   if (n <= 0) return ZQI_OK;
   if (nb < 1 || nb > NBMAX || w.tsm < 0 || w.tsm > TSM_MAX) return ZQI_ERR_ARG;
@@ -703,8 +690,6 @@ int reduce(quat* M, int n, int ld, int nb, double* d, quat* e, const ReduceWork&
     for (int j = 0; j < wdt; ++j) {
       const int c = p + j;
       const int nblk = cdiv(n - c, RQ);            // row blocks from c
-      if (j == 0)
-        hipLaunchKernelGGL(k_col_norm, dim3(cdiv(n - c, 256)), dim3(256), 0, s, M, ld, n, c, w.part, ZB{nullptr, 0});
       const int sz = n - c - 1;                    // trailing size
       const int G = cdiv(sz, SG);
       const int spg = matvec_spg(sz, w.tsm);
@@ -803,8 +788,6 @@ int reduce_batched(int nprob, const int* nh, ZB zb, quat* M, int ld, int nb, dou
     const int p = (c / nb) * nb, j = c - p;
     const int nrow = L - c;
     const int nblk = cdiv(nrow, RQ);
-    if (j == 0)
-      hipLaunchKernelGGL(k_col_norm, dim3(cdiv(nrow, 256), 1, act), dim3(256), 0, s, M, ld, L, c, w.part, zb);
     const int sz = L - c - 1;
     const int NF = j > 0 ? (8 * j + 63) / 64 : 1;
     // each problem's matvec takes the tile shape of its own trailing size:
