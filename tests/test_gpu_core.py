@@ -61,9 +61,16 @@ def test_tridiagonal_matches_oracle(gpu_ctx, n):
         assert np.abs(e - np.abs(e0)).max() <= 1e-13 * scale * n
 
 
+# mirror: a palindromic tridiagonal (d and e read the same backwards), whose
+# two halves have the same spectrum after the middle tear, so the top merge
+# pairs every value of one half with an equal one of the other: deflation
+# rotations across the halves, the kept columns mixing both (the merge
+# product's both-halves group).  glued: copies of Wilkinson's W21+ glued by
+# 1e-8, clusters of nearly equal eigenvalues spread over the tree.
 @pytest.mark.parametrize("n,kind", [(1, "rand"), (2, "rand"), (7, "rand"), (64, "rand"), (65, "rand"),
                                     (130, "rand"), (300, "rand"), (257, "wilk"), (200, "clust"),
-                                    (190, "split"), (128, "zero")])
+                                    (190, "split"), (128, "zero"), (256, "mirror"), (512, "mirror"),
+                                    (336, "glued")])
 def test_tridiag_eig(gpu_ctx, n, kind):
     import torch
     import scipy.linalg.lapack as L
@@ -77,6 +84,14 @@ def test_tridiag_eig(gpu_ctx, n, kind):
         d = 1.0 + 1e-12 * rng.normal(size=n); e = 1e-10 * np.abs(rng.normal(size=n - 1))
     elif kind == "split":
         d = rng.normal(size=n); e = np.abs(rng.normal(size=n - 1)); e[::5] = 0.0
+    elif kind == "mirror":
+        h = n // 2
+        d1 = rng.normal(size=h); e1 = np.abs(rng.normal(size=h - 1))
+        d = np.concatenate([d1, d1[::-1]]); e = np.concatenate([e1, [0.5], e1[::-1]])
+    elif kind == "glued":
+        w = np.abs(np.arange(21) - 10.0)
+        d = np.tile(w, n // 21 + 1)[:n]; e = np.ones(n - 1)
+        e[20::21] = 1e-8
     else:
         d = np.zeros(n); e = np.zeros(n - 1)
     dd = torch.from_numpy(d.copy()).cuda()
