@@ -812,7 +812,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "single solve, 2n=8192 complex128, A Hermitian / B antisymmetric U(-1,1), "
+        "config": {"workload": f"single solve, 2n={n2} complex128, A Hermitian / B antisymmetric U(-1,1), "
                                "eigenvalues + Kramers-paired eigenvectors",
                    "n2": n2, "panel_width": nb, "panel_width_autotune_ms": tune,
                    "tile_shape": {"matvec_switch": tsm, "gemm64_tiles_per_cu": t64}, "seed": args.seed,
