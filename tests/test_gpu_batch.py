@@ -50,9 +50,10 @@ def _batch(ctx, Ds, jobz="V", want_info=True):
     return rc, info, [t.T.cpu().numpy() for t in Dts], [e[: D.shape[1] // 2].cpu().numpy() for e, D in zip(eigs, Ds)]
 
 
-# uneven sizes across both panel-width groups (n <= 256 and n > 256), with the
-# smallest cases (n = 1, 2) and sizes on either side of the 64-row D&C leaf
-SIZES = [300, 1, 65, 257, 2, 64, 130, 33, 256, 3, 200, 7]
+# uneven sizes on both sides of the back-transform block switch (n = 448 |
+# 449: two groups), the smallest cases (n = 1, 2) and sizes about the 16-row
+# D&C leaves and the 64-row reduction tail
+SIZES = [300, 1, 65, 257, 2, 64, 130, 33, 256, 3, 200, 7, 449, 17, 448]
 
 
 def test_batch_equals_solo_bitwise(gpu_ctx):
