@@ -234,21 +234,44 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   for (int cc = 0; cc < SCW; ++cc) xc[cc] = qload(M + min(C0 + SCW * w + cc, n - 1) + (size_t)c * ld);
   const quat ua = (ra < n) ? xa : qzero();
   double qfl = 0.0;   // this thread's share of x^* M x over the tile
-  if (active) {
+  // BF (the wide tiles): branch-free over the tile -- selects instead of
+  // divergent ifs, and an inactive tile computes zeros from its clamped loads
+  // and stores nothing -- so the body is one scheduling region: the
+  // wave-uniform loads of x at the tile's columns issue alongside the tile
+  // loads instead of after their wait, and the kernel needs 64 VGPRs instead
+  // of 84 (four 512-thread blocks per CU instead of two).  2n = 8192 solve
+  // -1.5 ms.  The narrow tiles keep the branches: branch-free they made the
+  // 512-problem batch 1.3% slower (also when held to five blocks per CU, their
+  // residency with branches), for 0.7% on single solves of 2n <= 1024.
+  constexpr bool BF = SW == 32;
+  if (BF || active) {
+    const double nua = BF ? qnorm2(ua) : 0.0;
     quat acc = qzero();
     double col[4 * SCW];
 #pragma unroll
     for (int cc = 0; cc < SCW; ++cc) {
       const int cl = SCW * w + cc;
       const int cg = C0 + cl;
-      quat a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
-      if (ra == cg) {
-        a = quat{a.ar, 0.0, 0.0, 0.0};
-        qfl -= a.ar * qnorm2(ua);   // the diagonal enters x^* M x once, below twice
+      const bool in = cg < n && ra < n && ra >= cg;
+      const bool dg = ra == cg;   // the diagonal: real, enters x^* M x once (below twice)
+      quat a, cp = qzero();
+      if constexpr (BF) {
+        a = quat{in ? qa[cc].ar : 0.0, (in && !dg) ? qa[cc].ai : 0.0, (in && !dg) ? qa[cc].br : 0.0,
+                 (in && !dg) ? qa[cc].bi : 0.0};
+        qfl -= dg ? a.ar * nua : 0.0;
+        qfma(acc, a, xc[cc]);   // a = 0 where cg >= n
+        qfma_cl(cp, a, ua);
+        const bool below = ra > cg;
+        cp = quat{below ? cp.ar : 0.0, below ? cp.ai : 0.0, below ? cp.br : 0.0, below ? cp.bi : 0.0};
+      } else {
+        a = (cg < n && ra < n && ra >= cg) ? qa[cc] : qzero();
+        if (ra == cg) {
+          a = quat{a.ar, 0.0, 0.0, 0.0};
+          qfl -= a.ar * qnorm2(ua);
+        }
+        qfma(acc, a, cg < n ? xc[cc] : qzero());
+        if (ra > cg) qfma_cl(cp, a, ua);
       }
-      qfma(acc, a, cg < n ? xc[cc] : qzero());
-      quat cp = qzero();
-      if (ra > cg) qfma_cl(cp, a, ua);
       col[4 * cc + 0] = cp.ar; col[4 * cc + 1] = cp.ai; col[4 * cc + 2] = cp.br; col[4 * cc + 3] = cp.bi;
     }
     red[w][0][lane] = acc.ar;
@@ -260,7 +283,7 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
     // lane l ends with value vi = l / LPV (component vi & 3 of column vi >> 2), summed over the wave
     constexpr int LPV = 64 / (4 * SCW);
     const double cs = butterfly_sum<4 * SCW>(col);
-    if (lane % LPV == 0) {
+    if (active && lane % LPV == 0) {
       const int vi = lane / LPV;
       const int cg = C0 + SCW * w + (vi >> 2);
       if (cg < n) reinterpret_cast<double*>(Pcol + (size_t)g * ldp + cg)[vi & 3] = cs;
