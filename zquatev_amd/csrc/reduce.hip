@@ -362,10 +362,11 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
   }
   const quat xi = qload(M + ic + (size_t)c * ld);
   const quat ms0raw = qload(M + ic + (size_t)s0 * ld);   // stale column s0 = c+1 (row i)
-  const double m00 = scal[9];   // M[s0, s0] itself is rewritten by this grid (row s0 of column s0)
-  const double gam = scal[0];
-  const quat u1 = quat{scal[1], scal[2], scal[3], scal[4]};
-  const quat x1 = quat{scal[5], scal[6], scal[7], scal[8]};
+  // The column scalars (gamma, u1, x1, M[s0, s0]) in one vector load over
+  // lanes 0..9, taken apart after the partial gathers.  As scalar loads they
+  // would share the scalar-load counter with the kernel arguments the gather
+  // loop needs, and the loop's loads could not issue before they arrived.
+  const double sv = scal[min(lane, 9)];
   quat al = qzero(), bl = qzero(), cwl = qzero(), cvl = qzero();
   if (tid < j) {
     al = qload(TS + tid);
@@ -414,6 +415,10 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(WPE))) 
     for (int u = 0; u < PQ; ++u) qf += pq[u];
     w0 = qadd(w0, pw);
   }
+  const double gam = lane_d(sv, 0);
+  const quat u1 = quat{lane_d(sv, 1), lane_d(sv, 2), lane_d(sv, 3), lane_d(sv, 4)};
+  const quat x1 = quat{lane_d(sv, 5), lane_d(sv, 6), lane_d(sv, 7), lane_d(sv, 8)};
+  const double m00 = lane_d(sv, 9);   // M[s0, s0] itself is rewritten by this grid (row s0 of column s0)
   // ---- block sums: x^* M x, Re(a^* b), (M x)_{s0}, V(s0,:) a + W(s0,:) b
   {
     quat rd = qmul(cvl, al);

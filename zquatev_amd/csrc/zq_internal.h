@@ -118,6 +118,10 @@ __device__ __forceinline__ unsigned hi32(double v) {
 __device__ __forceinline__ double mkd(unsigned lo, unsigned hi) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+// value of lane k (wave-uniform k) of a per-lane double, in scalar registers
+__device__ __forceinline__ double lane_d(double v, int k) {
+  return mkd((unsigned)__builtin_amdgcn_readlane((int)lo32(v), k), (unsigned)__builtin_amdgcn_readlane((int)hi32(v), k));
+}
 template <int CTRL>
 __device__ __forceinline__ double dppd(double v) {
   return mkd(__builtin_amdgcn_update_dpp(0u, lo32(v), CTRL, 0xF, 0xF, false),
