@@ -48,6 +48,10 @@ constexpr int GBN = 64, GBK = 8;
 // orientation (8 consecutive k per 8 threads) fall in distinct banks.
 constexpr int BSK = GBN * 4 + 2;   // B k-slice stride
 
+// Operand elements as stored (the conjugation of a CT operand is applied
+// when the stage is stashed in LDS, after the current stage's MFMAs: applied
+// here, right after the load, it made every prefetch wait for its own data
+// before those MFMAs could start).
 template <bool CT>
 __device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, int K) {
   if (i >= m || k >= K) return qzero();
@@ -56,8 +60,7 @@ __device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, i
   if (k >= A.ksplit) { p = A.p1; ldp = A.ld1; kk = k - A.ksplit; }
   const int sr = CT ? kk : i, sc = CT ? i : kk;
   if (!(sr > sc + A.mask_off)) return qzero();
-  const quat v = qload_g(p + sr + (size_t)sc * ldp);
-  return CT ? qconj(v) : v;
+  return qload_g(p + sr + (size_t)sc * ldp);
 }
 
 template <bool CT>
@@ -68,8 +71,7 @@ __device__ __forceinline__ quat ld_opB(const QOperand& B, int k, int j, int K, i
   if (k >= B.ksplit) { p = B.p1; ldp = B.ld1; kk = k - B.ksplit; }
   const int sr = CT ? j : kk, sc = CT ? kk : j;
   if (!(sr > sc + B.mask_off)) return qzero();
-  const quat v = qload_g(p + sr + (size_t)sc * ldp);
-  return CT ? qconj(v) : v;
+  return qload_g(p + sr + (size_t)sc * ldp);
 }
 
 __device__ __forceinline__ double flip_sign(double v, unsigned mask) {
@@ -121,17 +123,19 @@ __device__ __forceinline__ void qgemm2_tile(const QGemmTask& t, int row0, int co
   auto stash = [&](int buf) {
 #pragma unroll
     for (int h = 0; h < AH; ++h) {
+      const quat a = ACT ? qconj(ra[h]) : ra[h];
       double2* pa = reinterpret_cast<double2*>(&As[buf][ak[h] * AK + ai[h] * 4]);
-      pa[0] = make_double2(ra[h].ar, ra[h].ai);
-      pa[1] = make_double2(ra[h].br, ra[h].bi);
+      pa[0] = make_double2(a.ar, a.ai);
+      pa[1] = make_double2(a.br, a.bi);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      const quat bv = BCT ? qconj(rb[h]) : rb[h];
       double* pb = &Bs[buf][bk[h] * BSK + bjj[h]];
-      pb[0] = rb[h].ar;
-      pb[GBN] = rb[h].ai;
-      pb[2 * GBN] = rb[h].br;
-      pb[3 * GBN] = rb[h].bi;
+      pb[0] = bv.ar;
+      pb[GBN] = bv.ai;
+      pb[2 * GBN] = bv.br;
+      pb[3 * GBN] = bv.bi;
     }
   };
   d4 acc[NA][2];
