@@ -48,10 +48,10 @@ constexpr int GBN = 64, GBK = 8;
 // orientation (8 consecutive k per 8 threads) fall in distinct banks.
 constexpr int BSK = GBN * 4 + 2;   // B k-slice stride
 
-// Operand elements as stored (the conjugation of a CT operand is applied
-// when the stage is stashed in LDS, after the current stage's MFMAs: applied
-// here, right after the load, it made every prefetch wait for its own data
-// before those MFMAs could start).
+// Operand elements as stored.  The conjugation of a CT operand is applied
+// when the stage is stashed in LDS, after the current stage's MFMAs: sign
+// flips here, on the freshly loaded registers, would make every prefetch
+// wait for its own data before those MFMAs could start.
 template <bool CT>
 __device__ __forceinline__ quat ld_opA(const QOperand& A, int i, int k, int m, int K) {
   if (i >= m || k >= K) return qzero();
