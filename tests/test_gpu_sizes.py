@@ -44,6 +44,25 @@ def test_tridiagonal_panel_widths(gpu_ctx, nb):
     assert np.abs(e - np.abs(e0)).max() <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("tsm,n", [(0, 300), (150, 300), (0, 97), (0, 130)])
+def test_tridiagonal_matvec_switch(gpu_ctx, tsm, n):
+    """The matvec tile shapes against the oracle.  Switch 0 runs the wide
+    (64 x 32, branch-free) tiles at every trailing size, including the partly
+    empty last row groups of small trailing sizes (97 and 130 leave 32 and 1
+    rows in the last group at the start); 150 changes shape mid-solve."""
+    _say(f"case: tridiagonalize n={n} matvec switch {tsm}")
+    D = gen.uniform_case(n, 4100 + n + tsm)
+    d0, e0, _, _ = zq_oracle.reduce_to_tridiagonal(D[:n, :n], D[n:, :n])
+    gpu_ctx.set_tile_shape(tsm, 0)
+    try:
+        d, e = _tridiag_gpu(gpu_ctx, D, n)
+    finally:
+        gpu_ctx.set_tile_shape(-1, 0)
+    scale = np.linalg.norm(gen.full_hermitian(D, n), 2)
+    assert np.abs(d - d0).max() <= 1e-12 * scale
+    assert np.abs(e - np.abs(e0)).max() <= 1e-12 * scale
+
+
 @pytest.mark.parametrize("n", [1000, 2048, 4096])
 def test_tridiag_eig_large(gpu_ctx, n):
     import torch
