@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--tile64", type=int, default=0,
                     help="GEMM 64-row tiles from this many 64x64 tiles per CU (0: autotuned / default)")
     ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--diag-shift", type=float, default=0.0,
+                    help="single: added to the diagonal of A (the 2n = 32768 large-solve case uses 10)")
     ap.add_argument("--cpu-n2", type=int, default=4096,
                     help="size of the bounded CPU-baseline sample (median of --cpu-reps runs on the host cores)")
     ap.add_argument("--cpu-reps", type=int, default=3)
@@ -596,10 +598,18 @@ def main():
 
     n2 = args.n2
     n = n2 // 2
-    D = gen.uniform_case(n, args.seed)
     dev = torch.device("cuda:0")
-    # row-major D^T == column-major D
-    Dt = torch.from_numpy(np.ascontiguousarray(D.T)).to(dev)
+    if args.profile_run and n2 > 8192:
+        # large profile runs: the same matrix generated on the device, lower
+        # triangles of A and B only (what the solver reads; the host copy and
+        # the checks below are not run here)
+        D = None
+        Dt = torch.zeros((n2, n2), dtype=torch.complex128, device=dev)
+        gen.uniform_case_device(n, args.seed, Dt, diag_shift=args.diag_shift)
+    else:
+        D = gen.uniform_case(n, args.seed, diag_shift=args.diag_shift)
+        # row-major D^T == column-major D
+        Dt = torch.from_numpy(np.ascontiguousarray(D.T)).to(dev)
     pristine = Dt[:n, :].clone()          # left half (first n columns of D)
     eig = torch.zeros(n, dtype=torch.float64, device=dev)
     ctx = zq.Context(0, panel_width=args.nb)
