@@ -221,11 +221,14 @@ __global__ void __launch_bounds__(SYT) k_symv(const quat* __restrict__ M, int ld
   const bool active = C0 < n && R0 < n;   // uniform for the block
   const int ra = R0 + lane;
   const int rac = min(ra, n - 1);
+  // Above the diagonal (row < column: unused, masked below) a lane reads the
+  // column's diagonal entry instead, a line the tile fetches anyway, so the
+  // diagonal tiles' upper halves cost no memory traffic and no branch.
   quat qa[SCW];
 #pragma unroll
   for (int cc = 0; cc < SCW; ++cc) {
     const int cgc = min(C0 + SCW * w + cc, n - 1);
-    qa[cc] = qload(M + rac + (size_t)cgc * ld);
+    qa[cc] = qload(M + max(rac, cgc) + (size_t)cgc * ld);
   }
   const quat xa = qload(M + rac + (size_t)c * ld);
   // x at this wave's 4 columns (the same address across the wave: one fetch each)
